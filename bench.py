@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Batched FFT throughput on one MI355X, through the asdFft C ABI.
+
+One step = one pass over the plain-FP32 coverage workload, all forward except
+where noted, inputs resident in HBM:
+    1D C2C complex64 N in {1000, 2187, 6000, 4096}, batch 256 each
+    1D R2C float32 N=8192 batch 1024, then C2R (inverse) of that spectrum
+    2D C2C 2048 x 2048, batch 4
+    3D C2C 256^3
+Inputs are synthetic uniform[-1, 1] (the R2C input is Gaussian noise plus three
+random tones per row), generated on the device from fixed seeds.
+
+value = aggregate GFLOP/s over the step, FLOP = 5 N log2 N per transform
+(N = total points; the same formula for R2C/C2R).  Timing uses HIP events on
+the stream the plans launch on, bracketed by device synchronisation.  The JSON
+line also carries the roofline of the dominant kernel launch (algorithmic
+bytes / its average event-timed duration vs 8 TB/s HBM) and the CPU FP64
+oracle's throughput on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "batched FFT GFLOP/s + % of HBM roofline; served FFT p50/p99 latency"
+
+# (name, kind, dims, batch, inverse, seed)
+WORKLOAD = [
+    ("c2c_n1000_b256", "c2c", (1000,), 256, False, 1),
+    ("c2c_n2187_b256", "c2c", (2187,), 256, False, 2),
+    ("c2c_n6000_b256", "c2c", (6000,), 256, False, 3),
+    ("c2c_n4096_b256", "c2c", (4096,), 256, False, 4),
+    ("r2c_n8192_b1024", "r2c", (8192,), 1024, False, 5),
+    ("c2r_n8192_b1024", "c2r", (8192,), 1024, True, 5),
+    ("c2c2d_2048x2048_b4", "c2c", (2048, 2048), 4, False, 6),
+    ("c2c3d_256x256x256_b1", "c2c", (256, 256, 256), 1, False, 7),
+]
+
+
+def flops(dims, batch):
+    n = math.prod(dims)
+    return 5.0 * n * math.log2(n) * batch
+
+
+def alg_bytes(kind, dims, batch):
+    n = math.prod(dims)
+    if kind == "c2c":
+        return 16.0 * n * batch
+    half = math.prod(dims[:-1]) * (dims[-1] // 2 + 1)
+    return (4.0 * n + 8.0 * half) * batch
+
+
+def make_inputs(torch, dev, kind, dims, batch, seed):
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    shape = (batch,) + tuple(dims)
+    if kind == "c2c":
+        re = torch.rand(shape, generator=g, device=dev) * 2 - 1
+        im = torch.rand(shape, generator=g, device=dev) * 2 - 1
+        return torch.complex(re, im).contiguous()
+    if kind == "r2c":
+        n = dims[-1]
+        x = torch.randn(shape, generator=g, device=dev, dtype=torch.float64)
+        t = torch.arange(n, device=dev, dtype=torch.float64)
+        for _ in range(3):
+            k = torch.randint(1, n // 2, (batch, 1), generator=g, device=dev).double()
+            a = torch.rand((batch, 1), generator=g, device=dev, dtype=torch.float64) * 3.5 + 0.5
+            ph = torch.rand((batch, 1), generator=g, device=dev, dtype=torch.float64) * 2 * math.pi
+            x = x + a * torch.cos(2 * math.pi * k * t / n + ph)
+        return x.float().contiguous()
+    raise ValueError(kind)
+
+
+def event_time_ms(torch, stream, fn, reps):
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(reps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """FP64 oracle (single-threaded C) on a bounded sample of the same workload."""
+    import numpy as np
+
+    from oracle import oracle
+    oracle.build()
+    rng = np.random.default_rng(0)
+    # sample: 8 transforms of each 1D shape, one 2D 2048^2 slice, one 256^3 volume
+    sample = [("c2c", (1000,), 8), ("c2c", (2187,), 8), ("c2c", (6000,), 8), ("c2c", (4096,), 8),
+              ("r2c", (8192,), 8), ("c2r", (8192,), 8), ("c2c", (2048, 2048), 1), ("c2c", (256, 256, 256), 1)]
+    total_flop = 0.0
+    t0 = time.perf_counter()
+    for kind, dims, b in sample:
+        shape = (b,) + dims
+        if kind == "c2c":
+            x = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+            oracle.c2c_nd(x, len(dims), False)
+        elif kind == "r2c":
+            oracle.r2c(rng.standard_normal(shape).astype(np.float32))
+        else:
+            h = (b, dims[-1] // 2 + 1)
+            x = (rng.standard_normal(h) + 1j * rng.standard_normal(h)).astype(np.complex64)
+            oracle.c2r(x, dims[-1], True)
+        total_flop += flops(dims, b)
+        if time.perf_counter() - t0 > seconds_budget:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(total_flop / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": "FP64 C oracle, 1 thread: 8 transforms each of C2C N=1000/2187/6000/4096, R2C and C2R "
+                      "N=8192, plus one 2048x2048 2D and one 256^3 3D transform; "
+                      f"{total_flop / 1e9:.2f} GFLOP in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--kernel-reps", type=int, default=20, help="event-timed repetitions per kernel launch")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--only", default=None, help="run one workload entry by name (profiling)")
+    ap.add_argument("--verbose", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is a design note, "
+              "not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from sip_amd import _native as N
+    from sip_amd.fft import Plan
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    L = N.lib()
+    entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
+               (a.only == "c2r_n8192_b1024" and w[0] == "r2c_n8192_b1024")]
+
+    jobs = []
+    with torch.cuda.stream(stream):
+        prev_out = {}
+        for name, kind, dims, batch, inv, seed in entries:
+            plan = Plan(kind, dims, batch, inv)
+            if kind == "c2r":
+                x = prev_out["r2c_n8192_b1024"]
+            else:
+                x = make_inputs(torch, dev, kind, dims, batch, seed)
+            out_dtype = torch.float32 if kind == "c2r" else torch.complex64
+            y = torch.empty(plan.out_shape(), dtype=out_dtype, device=dev)
+            plan.bind_workspace(dev)
+            plan.bind_stream(stream.cuda_stream)
+            tin = N.AclTensor(x.shape, N.ACL_FLOAT if x.dtype == torch.float32 else N.ACL_COMPLEX64, x.data_ptr())
+            tout = N.AclTensor(y.shape, N.ACL_FLOAT if y.dtype == torch.float32 else N.ACL_COMPLEX64, y.data_ptr())
+            fn = {"c2c": L.asdFftExecC2C, "r2c": L.asdFftExecR2C, "c2r": L.asdFftExecC2R}[kind]
+            jobs.append(dict(name=name, kind=kind, dims=dims, batch=batch, plan=plan, x=x, y=y, tin=tin,
+                             tout=tout, fn=fn, flop=flops(dims, batch), bytes=alg_bytes(kind, dims, batch),
+                             steps=plan.step_bytes(), describe=plan.describe()))
+            prev_out[name] = y
+    torch.cuda.synchronize()
+
+    def run_job(j):
+        st = j["fn"](j["plan"].handle, j["tin"].ptr, j["tout"].ptr)
+        if st != 0:
+            raise RuntimeError(f"{j['name']}: exec returned {st}")
+
+    def run_step():
+        for j in jobs:
+            run_job(j)
+
+    with torch.cuda.stream(stream):
+        for _ in range(a.warmup):
+            run_step()
+        torch.cuda.synchronize()
+        t_wall0 = time.perf_counter()
+        ms_per_step = event_time_ms(torch, stream, run_step, a.steps)
+        wall_ms = (time.perf_counter() - t_wall0) * 1e3 / a.steps
+
+        # per-workload and per-launch timings (same stream, same kernels)
+        breakdown = []
+        launches = []
+        for j in jobs:
+            ms = event_time_ms(torch, stream, lambda: run_job(j), a.kernel_reps)
+            breakdown.append({"name": j["name"], "us": round(ms * 1e3, 2),
+                              "gflops": round(j["flop"] / ms / 1e6, 1),
+                              "alg_GBps": round(j["bytes"] / ms / 1e6, 1),
+                              "hbm_frac": round(j["bytes"] / ms / 1e6 / HBM_PEAK_GBS, 3)})
+            for si, sb in enumerate(j["steps"]):
+                def one(si=si):
+                    st = L.asdFftExtExecStep(j["plan"].handle, j["tin"].ptr, j["tout"].ptr, si)
+                    if st != 0:
+                        raise RuntimeError(f"{j['name']} step {si}: {st}")
+                kms = event_time_ms(torch, stream, one, a.kernel_reps)
+                launches.append({"workload": j["name"], "step": si, "us": kms * 1e3, "bytes": sb,
+                                 "GBps": sb / kms / 1e6})
+    total_flop = sum(j["flop"] for j in jobs)
+    value = total_flop / ms_per_step / 1e6  # GFLOP/s
+
+    dom = max(launches, key=lambda r: r["us"])
+    roofline = {"bound": "hbm", "achieved": round(dom["GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"{dom['workload']} launch {dom['step']}", "us": round(dom["us"], 2),
+                "algorithmic_bytes": dom["bytes"]}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain-fp32-coverage: C2C 1D N={1000,2187,6000,4096} b256; R2C then C2R N=8192 "
+                               "b1024; C2C 2D 2048x2048 b4; C2C 3D 256^3" if a.only is None else a.only,
+                   "parallelism": "single-gpu",
+                   "alg_bytes_per_step": sum(j["bytes"] for j in jobs),
+                   "step_hbm_frac": round(sum(j["bytes"] for j in jobs) / ms_per_step / 1e6 / HBM_PEAK_GBS, 4),
+                   "wall_ms_per_step": round(wall_ms, 4)},
+        "breakdown": breakdown,
+        "roofline": roofline,
+    }
+    if a.verbose:
+        line["launches"] = [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()} for r in launches]
+        for j in jobs:
+            print(j["describe"], file=sys.stderr)
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    for j in jobs:
+        j["tin"].close()
+        j["tout"].close()
+        j["plan"].close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

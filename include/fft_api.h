@@ -1,0 +1,158 @@
+/*
+ * fft_api.h -- batched FFT plan / exec interface of the MI355X (gfx950) FFT engine.
+ *
+ * Drop-in for the reference header include/fft_api.h (namespace AsdSip, extern "C"):
+ * same function names, enum values, argument order and status codes, so a C++
+ * caller of the reference recompiles unchanged and links against libasdfft.so.
+ * In C++ the out-parameters are references, exactly as in the reference; a C
+ * caller (or an FFI such as ctypes) sees the same symbols with pointers, which
+ * is the same ABI.
+ *
+ * Each declaration cites the reference declaration it replaces.
+ *
+ * Semantics (no normalisation in either direction; reference
+ * docs/en/API_Reference/FFT/FFT_1D.md and the unit-test goldens):
+ *   C2C FORWARD  X_k = sum x_n e^{-2 pi i nk/N}        C2C INVERSE  sum x_n e^{+2 pi i nk/N}
+ *   R2C FORWARD  rfft (N reals -> N/2+1 bins)           R2C INVERSE  conj(rfft)
+ *   C2R FORWARD  hfft(n=N)                              C2R INVERSE  unnormalised irfft(n=N)
+ *   2D/3D        the same per axis; the real axis is the last one.
+ * Tensors: 1D (batch, N) [HORIZONTAL] or (N, batch) [VERTICAL]; 2D (batch, X, Y);
+ * 3D (batch, X, Y, Z); COMPLEX64 is interleaved (re, im) float32; data pointers
+ * are HIP device pointers.  Exec is asynchronous on the bound HIP stream.
+ */
+#ifndef ASDSIP_FFT_API_H
+#define ASDSIP_FFT_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#include "aspb_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aclTensor aclTensor;
+
+#ifdef __cplusplus
+namespace AsdSip {
+#define ASDFFT_OUT(T) T&
+#define ASDFFT_DEFAULT(v) = v
+#else
+#define ASDFFT_OUT(T) T*
+#define ASDFFT_DEFAULT(v)
+#endif
+
+/* include/fft_api.h:22 */
+typedef void* asdFftHandle;
+
+/* include/fft_api.h:24 */
+enum asdFftStatus { SUCCESS = 0, FAILED = 1 };
+
+/* include/fft_api.h:26-34 */
+enum asdFftType {
+    ASCEND_FFT_C2C = 0x10,
+    ASCEND_FFT_C2R = 0x11,
+    ASCEND_FFT_R2C = 0x12,
+    ASCEND_STFT_C2C = 0x20,
+    ASCEND_STFT_C2R = 0x21,
+    ASCEND_STFT_R2C = 0x22,
+    ASCEND_FFT_C2C_SEP = 0x30
+};
+
+/* include/fft_api.h:36-39 */
+enum asdFftDirection {
+    ASCEND_FFT_FORWARD = 0x10,
+    ASCEND_FFT_INVERSE = 0x11,
+};
+
+/* include/fft_api.h:41-44 */
+enum asdFft1dDimType {
+    ASCEND_FFT_HORIZONTAL = 0x10,
+    ASCEND_FFT_VERTICAL = 0x11,
+};
+
+#ifndef __cplusplus
+typedef enum asdFftType asdFftType;
+typedef enum asdFftDirection asdFftDirection;
+typedef enum asdFft1dDimType asdFft1dDimType;
+#endif
+
+/* Creates an empty plan handle.  Replaces include/fft_api.h:48 (impl core/fft/fft_api.cpp:1047-1065). */
+AspbStatus asdFftCreate(ASDFFT_OUT(asdFftHandle) handle);
+
+/* Binds a hipStream_t; NULL is rejected.  Replaces include/fft_api.h:51 (fft_api.cpp:1067-1082). */
+AspbStatus asdFftSetStream(asdFftHandle handle, void* stream);
+
+/* 1D plan; a handle can be initialised once.  Replaces include/fft_api.h:55-57 (fft_api.cpp:1124-1190). */
+AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, asdFftType fftType, asdFftDirection direction,
+                            int64_t batchSize, asdFft1dDimType dimType ASDFFT_DEFAULT(ASCEND_FFT_HORIZONTAL));
+
+/* Not part of this engine's path: returns ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:59-62. */
+AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor* input, const int64_t nFft,
+                               const int64_t hopLengthOpt, const int64_t winLengthOpt, const bool center,
+                               const bool normalized, const bool onesidedOpt, int64_t lengthOpt,
+                               const bool returnComplex);
+
+/* Replaces include/fft_api.h:65 (fft_api.cpp:1752-1788). */
+AspbStatus asdFftExecC2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output);
+
+/* Not part of this engine's path: returns ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:68-69. */
+AspbStatus asdFftExecC2CSeparated(asdFftHandle handle, const aclTensor* inputReal, const aclTensor* inputImag,
+                                  const aclTensor* outputReal, const aclTensor* outputImag);
+
+/* Replaces include/fft_api.h:72 (fft_api.cpp:1790-1816). */
+AspbStatus asdFftExecC2R(asdFftHandle handle, const aclTensor* input, const aclTensor* output);
+
+/* Replaces include/fft_api.h:75 (fft_api.cpp:1818-1844). */
+AspbStatus asdFftExecR2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output);
+
+/* Not part of this engine's path: returns ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:77-78. */
+AspbStatus asdFftExecIstft(asdFftHandle handle, const aclTensor* input, const aclTensor* windowOpt,
+                           const aclTensor* output);
+
+/* Frees the plan and its device tables.  Replaces include/fft_api.h:81 (fft_api.cpp:1234-1243). */
+AspbStatus asdFftDestroy(asdFftHandle handle);
+
+/* Replaces include/fft_api.h:83-84 (fft_api.cpp:1245-1289). */
+AspbStatus asdFftMakePlan2D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSizeY, asdFftType fftType,
+                            asdFftDirection direction, int32_t batchSize);
+
+/* Replaces include/fft_api.h:86-87 (fft_api.cpp:1291-1337). */
+AspbStatus asdFftMakePlan3D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSizeY, int64_t fftSizeZ,
+                            asdFftType fftType, asdFftDirection direction, int32_t batchSize);
+
+/* Bytes of caller-owned device workspace the plan needs (0 if none); also records the
+ * amount for the exec-time guard.  Replaces include/fft_api.h:89 (fft_api.cpp:1339-1360). */
+AspbStatus asdFftGetWorkspaceSize(asdFftHandle handle, ASDFFT_OUT(size_t) workspaceSize);
+
+/* Replaces include/fft_api.h:91 (fft_api.cpp:1362-1374). */
+AspbStatus asdFftSetWorkspace(asdFftHandle handle, void* workspace);
+
+/* hipStreamSynchronize on the bound stream.  Replaces include/fft_api.h:93 (fft_api.cpp:1376-1390). */
+AspbStatus asdFftSynchronize(asdFftHandle handle);
+
+/* Replaces include/fft_api.h:95 (fft_api.cpp:1392-1404). */
+AspbStatus asdFftGetType(asdFftHandle handle, ASDFFT_OUT(asdFftType) fftType);
+
+/* Not part of this engine's path: return ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:105-122. */
+AspbStatus asdFftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, const aclTensor* input,
+                       aclTensor* output, void* stream, void* workspace);
+AspbStatus asdIfftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, double scale, const aclTensor* input,
+                        aclTensor* output, void* stream, void* workspace);
+
+#ifdef __cplusplus
+}  // namespace AsdSip
+#endif
+
+#undef ASDFFT_OUT
+#undef ASDFFT_DEFAULT
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

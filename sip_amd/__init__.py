@@ -1,0 +1,6 @@
+"""MI355X (gfx950) batched FFT engine: drop-in for the SiP FFT plan/exec path.
+
+The product is libasdfft.so (C ABI in include/fft_api.h); this package builds it
+(sip_amd.build) and exposes a Python/torch host side (sip_amd.fft).
+"""
+from .fft import Plan, PlanCache, asd_fft_c2c, asd_fft_c2r, asd_fft_r2c  # noqa: F401

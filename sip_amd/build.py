@@ -1,0 +1,109 @@
+"""Builds libasdfft.so (HIP kernels for gfx950 + C++ host library) in-tree.
+
+    python -m sip_amd.build [--force] [--jobs N]
+
+Kernel instances compile in parallel (one translation unit per (variant, EMAX)
+entry of ASDFFT_FOR_EACH_INSTANCE); host files compile with g++ against the
+HIP runtime headers; everything links with hipcc into sip_amd/lib/libasdfft.so.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import re
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libasdfft.so")
+ARCH = os.environ.get("ASDFFT_ARCH", "gfx950")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    # packed FP32 instructions are kept out of the generated code on purpose
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INCLUDE}", f"-I{CSRC}",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+              f"-I{ROCM}/include", f"-I{INCLUDE}", f"-I{CSRC}"]
+HOST_SOURCES = ["acl_tensor.cpp", "twiddles.cpp", "planner.cpp", "fft_api.cpp"]
+KERNEL_HEADERS = ["stockham_impl.h", "fft_kernel_args.h", "kernels.h", "radix_consts.h"]
+
+
+def instances():
+    text = open(os.path.join(CSRC, "kernels.h")).read()
+    body = text[text.index("#define ASDFFT_FOR_EACH_INSTANCE"):]
+    return [(int(v), int(e)) for v, e in re.findall(r"X\((\d+),\s*(\d+)\)", body)]
+
+
+def _newer(target, deps):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in r.stdout.splitlines() if "packed-fp32-ops" not in line)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    subprocess.run([sys.executable, os.path.join(CSRC, "gen_radix_consts.py")], check=True)
+    hdr = [os.path.join(CSRC, h) for h in KERNEL_HEADERS]
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    tasks = []
+    objs = []
+    for v, e in instances():
+        obj = os.path.join(BUILD, f"stockham_v{v}_e{e}.o")
+        src = os.path.join(CSRC, "stockham_inst.hip")
+        objs.append(obj)
+        if force or _newer(obj, [src] + hdr):
+            tasks.append(["hipcc"] + HIPCC_FLAGS + [f"-DASDFFT_VARIANT={v}", f"-DASDFFT_EMAX={e}", "-c", src,
+                                                    "-o", obj])
+    direct_obj = os.path.join(BUILD, "dft_direct.o")
+    objs.append(direct_obj)
+    if force or _newer(direct_obj, [os.path.join(CSRC, "dft_direct.hip")] + hdr):
+        tasks.append(["hipcc"] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, "dft_direct.hip"), "-o", direct_obj])
+    host_hdr = [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    host_hdr += [os.path.join(INCLUDE, h) for h in os.listdir(INCLUDE) if h.endswith(".h")]
+    for f in HOST_SOURCES:
+        obj = os.path.join(BUILD, f.replace(".cpp", ".o"))
+        objs.append(obj)
+        if force or _newer(obj, [os.path.join(CSRC, f)] + host_hdr):
+            tasks.append(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, f), "-o", obj])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(_run, tasks):
+            if verbose and out.strip():
+                print(out)
+    if force or tasks or not os.path.exists(LIB):
+        tmp = LIB + ".tmp"
+        _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+             [f"-L{ROCM}/lib", "-lamdhip64", "-Wl,--no-undefined"])
+        os.replace(tmp, LIB)
+    return LIB
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("-v", "--verbose", action="store_true")
+    a = ap.parse_args()
+    print(build(force=a.force, jobs=a.jobs, verbose=a.verbose))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,649 @@
+// Implementation of include/fft_api.h on HIP.
+//
+// Validation order, status codes and plan-state rules follow the reference
+// implementation core/fft/fft_api.cpp (cited per function); the plans
+// themselves come from the gfx950 planner (planner.cpp) and run as HIP kernels
+// on the bound stream.  There is no host fallback: without a HIP device, plans
+// can be built and validated but every exec returns ACL_ERROR_INTERNAL_ERROR.
+#include "fft_api.h"
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "acl_meta.h"
+#include "asdfft_ext.h"
+#include "planner.h"
+#include "tensor_desc.h"
+
+namespace {
+
+using namespace AsdSip;
+
+constexpr int64_t kMaxFftSize = int64_t(1) << 27;  // fft_api.cpp:82
+
+enum class PlanState { kCreated, kInitialized, kFailed };
+
+struct FftPlan {
+    std::mutex mu;
+    PlanState state = PlanState::kCreated;
+    asdFftType type = ASCEND_FFT_C2C;
+    asdFftDirection direction = ASCEND_FFT_FORWARD;
+    std::vector<int64_t> sizes;
+    int64_t batch = 1;          // transforms per exec (horizontal / multi-dim)
+    int64_t vertical_batch = 1; // VERTICAL 1D: tensor (N, vertical_batch)
+    bool vertical = false;
+    void* stream = nullptr;
+    void* workspace = nullptr;
+    size_t workspace_recorded = 0;  // set by GetWorkspaceSize (fft_api.cpp:1339-1360)
+    bool workspace_queried = false;
+    asdfft::BuiltPlan built;
+};
+
+std::mutex g_table_mu;
+std::unordered_map<void*, std::shared_ptr<FftPlan>>& table()
+{
+    static auto* t = new std::unordered_map<void*, std::shared_ptr<FftPlan>>();
+    return *t;
+}
+
+std::shared_ptr<FftPlan> lookup(asdFftHandle h)
+{
+    std::lock_guard<std::mutex> lock(g_table_mu);
+    auto it = table().find(h);
+    if (it == table().end()) return nullptr;
+    return it->second;
+}
+
+void log_error(const char* fn, const std::string& msg)
+{
+    static const bool quiet = [] {
+        const char* e = getenv("ASDFFT_QUIET");
+        return e != nullptr && e[0] == '1';
+    }();
+    if (!quiet) fprintf(stderr, "[asdfft] %s: %s\n", fn, msg.c_str());
+}
+
+bool trace_enabled()
+{
+    const char* e = getenv("ASDFFT_PLAN_TRACE");
+    return e != nullptr && strcmp(e, "1") == 0;
+}
+
+bool valid_direction(asdFftDirection d)  // CheckFftDirection, fft_api.cpp:1088-1096
+{
+    return d == ASCEND_FFT_FORWARD || d == ASCEND_FFT_INVERSE;
+}
+
+asdfft::Kind kind_of(asdFftType t)
+{
+    if (t == ASCEND_FFT_R2C) return asdfft::Kind::R2C;
+    if (t == ASCEND_FFT_C2R) return asdfft::Kind::C2R;
+    return asdfft::Kind::C2C;
+}
+
+AspbStatus finish_plan(FftPlan& p, const char* fn)
+{
+    asdfft::PlanSpec spec;
+    spec.kind = kind_of(p.type);
+    spec.inverse = (p.direction == ASCEND_FFT_INVERSE);
+    spec.dims = p.sizes;
+    spec.batch = p.vertical ? p.vertical_batch : p.batch;
+    spec.vertical = p.vertical;
+    asdfft::BuiltPlan built;
+    std::string why;
+    try {
+        why = asdfft::build_plan(spec, &built);
+    } catch (const std::exception& e) {
+        why = e.what();
+    }
+    if (!why.empty()) {
+        log_error(fn, why);
+        p.built = asdfft::BuiltPlan{};
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    p.built = std::move(built);
+    p.state = PlanState::kInitialized;
+    if (trace_enabled()) fprintf(stderr, "[asdfft] %s", asdfft::describe_plan(p.built).c_str());
+    return ACL_SUCCESS;
+}
+
+// ---- tensor / plan matching (fft_api.cpp:1529-1722) -------------------------------------
+bool leading_extent_matches(const std::vector<int64_t>& dims, int64_t lead_count, int64_t batch)
+{
+    if (batch <= 0) return false;
+    int64_t ext = 1;
+    for (int64_t j = 0; j < lead_count; j++) {
+        if (dims[j] <= 0 || ext > batch / dims[j]) return false;
+        ext *= dims[j];
+    }
+    return ext == batch;
+}
+
+int64_t plan_batch_extent(const FftPlan& p) { return p.vertical ? p.vertical_batch : p.batch; }
+
+// matchC2C1D_ (fft_api.cpp:1549-1566)
+bool match_c2c_1d(const FftPlan& p, const std::vector<int64_t>& d)
+{
+    if (d.size() != 2) return false;
+    const int64_t n = p.sizes[0];
+    if (p.vertical && p.vertical_batch > 1) return d[0] == n && d[1] == p.vertical_batch;
+    if (plan_batch_extent(p) == 1) return (d[0] == 1 && d[1] == n) || (d[0] == n && d[1] == 1);
+    return d[0] == p.batch && d[1] == n;
+}
+
+// matchC2C_ (fft_api.cpp:1568-1618); multi-dimensional plans additionally require the
+// leading dimensions to cover exactly the plan's batch, as the real variants do.
+bool match_c2c(const FftPlan& p, const aclTensor* t)
+{
+    if (t == nullptr || t->dtype != ACL_COMPLEX64) return false;
+    const auto& d = t->view;
+    if (p.sizes.size() == 1) return match_c2c_1d(p, d);
+    if (d.size() < p.sizes.size()) return false;
+    size_t i = d.size() - 1;
+    for (size_t k = p.sizes.size(); k-- > 0; i--) {
+        if (d[i] != p.sizes[k]) return false;
+    }
+    return leading_extent_matches(d, (int64_t)(d.size() - p.sizes.size()), p.batch);
+}
+
+// matchC2R_ (fft_api.cpp:1620-1674): complex half spectrum
+bool match_half_complex(const FftPlan& p, const aclTensor* t)
+{
+    if (t == nullptr || t->dtype != ACL_COMPLEX64) return false;
+    const auto& d = t->view;
+    if (d.size() < p.sizes.size()) return false;
+    int64_t i = (int64_t)d.size() - 1;
+    for (size_t k = p.sizes.size(); k-- > 0; i--) {
+        const int64_t want = (i == (int64_t)d.size() - 1) ? p.sizes[k] / 2 + 1 : p.sizes[k];
+        if (d[i] != want) return false;
+    }
+    return leading_extent_matches(d, i + 1, plan_batch_extent(p));
+}
+
+// matchR2C_ (fft_api.cpp:1676-1722): real signal
+bool match_real(const FftPlan& p, const aclTensor* t)
+{
+    if (t == nullptr || t->dtype != ACL_FLOAT) return false;
+    const auto& d = t->view;
+    if (d.size() < p.sizes.size()) return false;
+    int64_t i = (int64_t)d.size() - 1;
+    for (size_t k = p.sizes.size(); k-- > 0; i--) {
+        if (d[i] != p.sizes[k]) return false;
+    }
+    return leading_extent_matches(d, i + 1, plan_batch_extent(p));
+}
+
+bool ranges_overlap(const void* a, size_t na, const void* b, size_t nb)
+{
+    const char* x = static_cast<const char*>(a);
+    const char* y = static_cast<const char*>(b);
+    return x < y + nb && y < x + na;
+}
+
+// asdFftExecV2 (fft_api.cpp:1446-1511): workspace guard, then the launch list
+// (only_step >= 0 launches that one step, for per-kernel timing).
+AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output, const char* fn, int only_step = -1)
+{
+    const asdfft::BuiltPlan& bp = p.built;
+    if (bp.workspace > 0 && p.workspace == nullptr) {  // CheckWorkspaceBinding, fft_api.cpp:1020-1032
+        log_error(fn, "workspace has not been set");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    if (bp.workspace > 0 && !p.workspace_queried) {
+        log_error(fn, "workspace size has not been queried, call asdFftGetWorkspaceSize before exec");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (bp.workspace > p.workspace_recorded) {
+        log_error(fn, "workspace smaller than the plan needs");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    void* in = asdfft::data_address(input);
+    void* out = asdfft::data_address(output);
+    if (in == nullptr || out == nullptr) {
+        log_error(fn, "tensor data is nullptr");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    if (!asdfft::is_dense(input) || !asdfft::is_dense(output)) {
+        log_error(fn, "only dense row-major tensors are supported");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const size_t in_bytes = (size_t)asdfft::numel(input) * (size_t)asdfft::element_bytes(input->dtype);
+    const size_t out_bytes = (size_t)asdfft::numel(output) * (size_t)asdfft::element_bytes(output->dtype);
+    const bool same = (in == out && in_bytes == out_bytes && p.type == ASCEND_FFT_C2C);
+    if (!same && ranges_overlap(in, in_bytes, out, out_bytes)) {
+        log_error(fn, "input and output overlap (only exact in-place C2C is supported)");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!asdfft::device_available()) {
+        log_error(fn, "no HIP device: this library has no host execution path");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    for (const asdfft::Step& s : bp.steps) {
+        if (!s.tables_ready) {
+            log_error(fn, "plan was built without a HIP device; create it again on a device");
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    hipStream_t stream = static_cast<hipStream_t>(p.stream);
+    char* bufs[3] = {static_cast<char*>(in), static_cast<char*>(out), static_cast<char*>(p.workspace)};
+    for (size_t si = 0; si < bp.steps.size(); si++) {
+        if (only_step >= 0 && (size_t)only_step != si) continue;
+        const asdfft::Step& s = bp.steps[si];
+        hipError_t e;
+        if (s.direct) {
+            asdfft::DftArgs a = s.dargs;
+            a.in = bufs[s.src] + s.src_offset;
+            a.out = bufs[s.dst] + s.dst_offset;
+            e = asdfft::launch_dft_direct(s.shape, a, stream);
+        } else {
+            asdfft::FftArgs a = s.args;
+            a.in = bufs[s.src] + s.src_offset;
+            a.out = bufs[s.dst] + s.dst_offset;
+            e = asdfft::launch_stockham(s.shape, a, stream);
+        }
+        if (e != hipSuccess) {
+            log_error(fn, std::string("kernel launch failed: ") + hipGetErrorString(e));
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    return ACL_SUCCESS;
+}
+
+template <typename MatchIn, typename MatchOut>
+AspbStatus exec_common(asdFftHandle handle, const aclTensor* input, const aclTensor* output, asdFftType want,
+                       MatchIn match_in, MatchOut match_out, const char* fn, int only_step = -1)
+{
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state != PlanState::kInitialized) {
+        log_error(fn, "plan is not initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (p.type != want) {  // fft_api.cpp:1764-1767
+        log_error(fn, "plan fftType does not match this exec entry point");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!match_in(p, input)) {
+        log_error(fn, "input does not match plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!match_out(p, output)) {
+        log_error(fn, "output does not match plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (only_step >= (int)p.built.steps.size()) {
+        log_error(fn, "step index out of range");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    return run_plan(p, input, output, fn, only_step);
+}
+
+}  // namespace
+
+namespace AsdSip {
+
+extern "C" {
+
+AspbStatus asdFftCreate(asdFftHandle& handle)  // fft_api.cpp:1047-1065
+{
+    try {
+        auto p = std::make_shared<FftPlan>();
+        void* key = p.get();
+        std::lock_guard<std::mutex> lock(g_table_mu);
+        table()[key] = std::move(p);
+        handle = key;
+    } catch (const std::exception& e) {
+        handle = nullptr;
+        log_error("asdFftCreate", e.what());
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftSetStream(asdFftHandle handle, void* stream)  // fft_api.cpp:1067-1082
+{
+    if (stream == nullptr) {
+        log_error("asdFftSetStream", "stream is nullptr");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error("asdFftSetStream", "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lock(sp->mu);
+    sp->stream = stream;
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, asdFftType fftType, asdFftDirection direction,
+                            int64_t batchSize, asdFft1dDimType dimType)  // fft_api.cpp:1124-1190
+{
+    const char* fn = "asdFftMakePlan1D";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (fftSize <= 0 || fftSize > kMaxFftSize) {
+        log_error(fn, "invalid fft_size");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (batchSize <= 0 || batchSize > std::numeric_limits<int32_t>::max()) {
+        log_error(fn, "invalid batch_size");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftType != ASCEND_FFT_C2C && fftType != ASCEND_FFT_C2R && fftType != ASCEND_FFT_R2C &&
+        fftType != ASCEND_FFT_C2C_SEP) {
+        log_error(fn, "unsupported fftType for a 1D plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!valid_direction(direction)) {
+        log_error(fn, "invalid direction");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (p.state == PlanState::kInitialized) {
+        log_error(fn, "plan is already initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (dimType != ASCEND_FFT_HORIZONTAL && dimType != ASCEND_FFT_VERTICAL) {  // Setup1dBatchLayout
+        log_error(fn, "invalid dimType");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftType == ASCEND_FFT_C2C_SEP) {
+        log_error(fn, "separated real/imag planes are not part of this engine");
+        return ACL_ERROR_API_NOT_SUPPORT;
+    }
+    if (dimType == ASCEND_FFT_VERTICAL && batchSize > 1 && fftType != ASCEND_FFT_C2C) {
+        log_error(fn, "VERTICAL layout with batch > 1 is supported for C2C only");
+        return ACL_ERROR_API_NOT_SUPPORT;
+    }
+    p.type = fftType;
+    p.direction = direction;
+    p.sizes = {fftSize};
+    p.vertical = (dimType == ASCEND_FFT_VERTICAL);
+    p.batch = p.vertical ? 1 : batchSize;
+    p.vertical_batch = p.vertical ? batchSize : 1;
+    if (p.vertical && batchSize == 1) p.vertical = false;  // (1, N) and (N, 1) both accepted
+    return finish_plan(p, fn);
+}
+
+AspbStatus asdFftMakePlan2D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSizeY, asdFftType fftType,
+                            asdFftDirection direction, int32_t batchSize)  // fft_api.cpp:1245-1289
+{
+    const char* fn = "asdFftMakePlan2D";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftSizeX <= 0 || fftSizeX > kMaxFftSize || fftSizeY <= 0 || fftSizeY > kMaxFftSize || batchSize <= 0) {
+        log_error(fn, "invalid size or batch");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftType != ASCEND_FFT_C2C && fftType != ASCEND_FFT_C2R && fftType != ASCEND_FFT_R2C) {
+        log_error(fn, "unsupported fftType for a 2D plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!valid_direction(direction)) {
+        log_error(fn, "invalid direction");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state == PlanState::kInitialized) {
+        log_error(fn, "plan is already initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const int64_t imax = std::numeric_limits<int32_t>::max();  // derived batch guard, fft_api.cpp:666-673
+    if ((int64_t)batchSize * fftSizeX > imax || (int64_t)batchSize * fftSizeY > imax) {
+        log_error(fn, "derived batch size exceeds INT32_MAX");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    p.type = fftType;
+    p.direction = direction;
+    p.sizes = {fftSizeX, fftSizeY};
+    p.batch = batchSize;
+    p.vertical = false;
+    p.vertical_batch = 1;
+    return finish_plan(p, fn);
+}
+
+AspbStatus asdFftMakePlan3D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSizeY, int64_t fftSizeZ,
+                            asdFftType fftType, asdFftDirection direction, int32_t batchSize)  // fft_api.cpp:1291-1337
+{
+    const char* fn = "asdFftMakePlan3D";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftSizeX <= 0 || fftSizeX > kMaxFftSize || fftSizeY <= 0 || fftSizeY > kMaxFftSize || batchSize <= 0 ||
+        fftSizeZ <= 0 || fftSizeZ > kMaxFftSize) {
+        log_error(fn, "invalid size or batch");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftType != ASCEND_FFT_C2C && fftType != ASCEND_FFT_C2R && fftType != ASCEND_FFT_R2C &&
+        fftType != ASCEND_FFT_C2C_SEP) {
+        log_error(fn, "unsupported fftType for a 3D plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!valid_direction(direction)) {
+        log_error(fn, "invalid direction");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state == PlanState::kInitialized) {
+        log_error(fn, "plan is already initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (fftType == ASCEND_FFT_C2C_SEP) {
+        log_error(fn, "separated real/imag planes are not part of this engine");
+        return ACL_ERROR_API_NOT_SUPPORT;
+    }
+    const int64_t imax = std::numeric_limits<int32_t>::max();
+    const int64_t b = batchSize;
+    if (fftSizeX > imax / b / fftSizeY || fftSizeX > imax / b / fftSizeZ || fftSizeY > imax / b / fftSizeZ) {
+        log_error(fn, "derived batch size exceeds INT32_MAX");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    p.type = fftType;
+    p.direction = direction;
+    p.sizes = {fftSizeX, fftSizeY, fftSizeZ};
+    p.batch = batchSize;
+    p.vertical = false;
+    p.vertical_batch = 1;
+    return finish_plan(p, fn);
+}
+
+AspbStatus asdFftGetWorkspaceSize(asdFftHandle handle, size_t& workspaceSize)  // fft_api.cpp:1339-1360
+{
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error("asdFftGetWorkspaceSize", "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lock(sp->mu);
+    workspaceSize = sp->built.workspace;
+    sp->workspace_recorded = workspaceSize;
+    sp->workspace_queried = true;
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftSetWorkspace(asdFftHandle handle, void* workspace)  // fft_api.cpp:1362-1374
+{
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error("asdFftSetWorkspace", "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lock(sp->mu);
+    sp->workspace = workspace;
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftSynchronize(asdFftHandle handle)  // fft_api.cpp:1376-1390
+{
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error("asdFftSynchronize", "invalid handle");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    void* s;
+    {
+        std::lock_guard<std::mutex> lock(sp->mu);
+        s = sp->stream;
+    }
+    if (!asdfft::device_available()) {
+        log_error("asdFftSynchronize", "no HIP device");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(s));
+    if (e != hipSuccess) {
+        log_error("asdFftSynchronize", hipGetErrorString(e));
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftGetType(asdFftHandle handle, asdFftType& fftType)  // fft_api.cpp:1392-1404
+{
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error("asdFftGetType", "invalid handle");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    std::lock_guard<std::mutex> lock(sp->mu);
+    fftType = sp->type;
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftDestroy(asdFftHandle handle)  // fft_api.cpp:1234-1243
+{
+    std::shared_ptr<FftPlan> victim;
+    {
+        std::lock_guard<std::mutex> lock(g_table_mu);
+        auto it = table().find(handle);
+        if (it == table().end()) {
+            log_error("asdFftDestroy", "invalid handle");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+        victim = std::move(it->second);
+        table().erase(it);
+    }
+    // wait for any in-flight exec on another thread before the tables go away
+    std::lock_guard<std::mutex> lock(victim->mu);
+    return ACL_SUCCESS;
+}
+
+AspbStatus asdFftExecC2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output)  // :1752-1788
+{
+    return exec_common(handle, input, output, ASCEND_FFT_C2C, match_c2c, match_c2c, "asdFftExecC2C");
+}
+
+AspbStatus asdFftExecC2R(asdFftHandle handle, const aclTensor* input, const aclTensor* output)  // :1790-1816
+{
+    return exec_common(handle, input, output, ASCEND_FFT_C2R, match_half_complex, match_real, "asdFftExecC2R");
+}
+
+AspbStatus asdFftExecR2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output)  // :1818-1844
+{
+    return exec_common(handle, input, output, ASCEND_FFT_R2C, match_real, match_half_complex, "asdFftExecR2C");
+}
+
+AspbStatus asdFftIstftMakePlan(asdFftHandle, const aclTensor*, const int64_t, const int64_t, const int64_t, const bool,
+                               const bool, const bool, int64_t, const bool)
+{
+    log_error("asdFftIstftMakePlan", "ISTFT is not part of this engine");
+    return ACL_ERROR_API_NOT_SUPPORT;
+}
+
+AspbStatus asdFftExecIstft(asdFftHandle, const aclTensor*, const aclTensor*, const aclTensor*)
+{
+    log_error("asdFftExecIstft", "ISTFT is not part of this engine");
+    return ACL_ERROR_API_NOT_SUPPORT;
+}
+
+AspbStatus asdFftExecC2CSeparated(asdFftHandle, const aclTensor*, const aclTensor*, const aclTensor*, const aclTensor*)
+{
+    log_error("asdFftExecC2CSeparated", "separated real/imag planes are not part of this engine");
+    return ACL_ERROR_API_NOT_SUPPORT;
+}
+
+AspbStatus asdFftShift(int64_t, int64_t, int64_t, int64_t, const aclTensor*, aclTensor*, void*, void*)
+{
+    log_error("asdFftShift", "fftshift is not part of this engine");
+    return ACL_ERROR_API_NOT_SUPPORT;
+}
+
+AspbStatus asdIfftShift(int64_t, int64_t, int64_t, int64_t, double, const aclTensor*, aclTensor*, void*, void*)
+{
+    log_error("asdIfftShift", "ifftshift is not part of this engine");
+    return ACL_ERROR_API_NOT_SUPPORT;
+}
+
+}  // extern "C"
+
+}  // namespace AsdSip
+
+// ---- extension entry points (asdfft_ext.h) -------------------------------------------------
+extern "C" int32_t asdFftExtDescribePlan(void* handle, char* buffer, size_t length, size_t* needed)
+{
+    auto sp = lookup(handle);
+    if (!sp) return AsdSip::ACL_ERROR_INVALID_PARAM;
+    std::lock_guard<std::mutex> lock(sp->mu);
+    const std::string s = asdfft::describe_plan(sp->built);
+    if (needed != nullptr) *needed = s.size() + 1;
+    if (buffer != nullptr && length > 0) {
+        const size_t n = std::min(length - 1, s.size());
+        memcpy(buffer, s.data(), n);
+        buffer[n] = '\0';
+    }
+    return AsdSip::ACL_SUCCESS;
+}
+
+extern "C" int32_t asdFftExtPlanSteps(void* handle, int32_t* steps, double* algorithmic_bytes)
+{
+    auto sp = lookup(handle);
+    if (!sp) return AsdSip::ACL_ERROR_INVALID_PARAM;
+    std::lock_guard<std::mutex> lock(sp->mu);
+    if (steps != nullptr) *steps = (int32_t)sp->built.steps.size();
+    if (algorithmic_bytes != nullptr) {
+        for (size_t i = 0; i < sp->built.steps.size(); i++) algorithmic_bytes[i] = sp->built.steps[i].bytes;
+    }
+    return AsdSip::ACL_SUCCESS;
+}
+
+extern "C" int32_t asdFftExtExecStep(void* handle, const aclTensor* input, const aclTensor* output, int32_t step)
+{
+    auto sp = lookup(handle);
+    if (!sp) return AsdSip::ACL_ERROR_INVALID_PARAM;
+    asdFftType t;
+    {
+        std::lock_guard<std::mutex> lock(sp->mu);
+        t = sp->type;
+    }
+    if (step < 0) return AsdSip::ACL_ERROR_INVALID_PARAM;
+    if (t == AsdSip::ASCEND_FFT_R2C)
+        return exec_common(handle, input, output, t, match_real, match_half_complex, "asdFftExtExecStep", step);
+    if (t == AsdSip::ASCEND_FFT_C2R)
+        return exec_common(handle, input, output, t, match_half_complex, match_real, "asdFftExtExecStep", step);
+    return exec_common(handle, input, output, t, match_c2c, match_c2c, "asdFftExtExecStep", step);
+}

@@ -1,0 +1,87 @@
+// Launch parameters shared by the host planner and the HIP kernels.
+//
+// One launch performs `count` independent transforms of length `n` (complex
+// points).  Transform t lives at
+//     base(t) = (t % cols) * inner + (t / cols) * outer      (element units)
+// for input and output separately, element i at base(t) + i * elem_stride.
+// Row batches use cols = 1; column passes of 2D/3D plans use cols = number of
+// adjacent columns, inner = 1.
+#pragma once
+#include <stdint.h>
+
+namespace asdfft {
+
+constexpr int kMaxPasses = 10;
+
+// Radix codelets compiled into the Stockham kernels.
+constexpr int kRadixSet[] = {16, 12, 10, 9, 8, 6, 4, 2, 19, 17, 13, 11, 7, 5, 3};
+
+// Thread mapping inside a workgroup.
+enum Mapping : int {
+    kMapRows = 0,  // lanes run along a transform (tid fastest): contiguous rows
+    kMapCols = 1,  // lanes run across transforms (transform fastest): strided columns
+};
+
+// How the first pass obtains its inputs.
+enum Prologue : int {
+    kProComplex = 0,   // complex load (conjugated when conj_in)
+    kProReal = 1,      // real load, imaginary part 0 (odd-length R2C)
+    kProHerm = 2,      // Hermitian half -> full spectrum (odd-length C2R)
+    kProC2REven = 3,   // even-length C2R pre-twiddle: Z[k] from X[k], X[M-k]
+};
+
+// How the last pass disposes of its outputs.
+enum Epilogue : int {
+    kEpiComplex = 0,   // complex store (conjugated when conj_out)
+    kEpiHalf = 1,      // store bins 0..n/2 only (odd-length R2C)
+    kEpiReal = 2,      // store the real part (odd-length C2R)
+    kEpiR2CEven = 3,   // even-length R2C post-twiddle through LDS
+    kEpiTwiddle = 4,   // multiply by W_big^(column * i), then store (four-step pass 1)
+    kEpiTranspose = 5, // store transposed through LDS (four-step pass 2)
+};
+
+struct PassDesc {
+    int radix;        // R
+    int span;         // L: product of the radices of the earlier passes
+    uint32_t magic;   // floor(2^32 / L) + 1 (L > 1): j / L == umulhi(j, magic)
+    int tw_offset;    // first entry of this pass in the twiddle table (layout [k][r-1])
+};
+
+struct FftArgs {
+    const void* in;
+    void* out;
+    const float* tw;        // per-pass twiddles, interleaved (re, im)
+    const float* tw_aux;    // R2C/C2R half-length twiddles W_N^k, k in [0, n]
+    const float* tw_big_lo; // four-step twiddles: W_big^m = hi[m >> big_shift] * lo[m & mask]
+    const float* tw_big_hi;
+    int n;                  // complex transform length
+    int threads;            // T: threads per transform
+    int tpb;                // transforms per workgroup
+    int passes;
+    int lds_tstride;        // row mapping: padded LDS words per transform
+    int lds_plane;          // LDS words per plane (re plane, then im plane)
+    int conj_in, conj_out;
+    int big_shift;          // four-step twiddle split
+    int pad_;
+    int64_t count;          // transforms in this launch
+    int64_t cols;
+    int64_t in_inner, in_outer, in_stride;
+    int64_t out_inner, out_outer, out_stride;
+    int64_t out_cols;       // transposed store: transforms per output group (kEpiTranspose)
+    PassDesc pass[kMaxPasses];
+};
+
+// Direct O(N^2) DFT kernel parameters (lengths with factors outside the radix set, N == 1).
+struct DftArgs {
+    const void* in;
+    void* out;
+    const float* roots;     // W_N^m, m in [0, N)
+    int n;
+    int tpb;
+    int conj_in, conj_out;
+    int prologue, epilogue; // kProComplex/kProReal/kProHerm, kEpiComplex/kEpiHalf/kEpiReal
+    int64_t count;
+    int64_t in_dist, out_dist;
+};
+
+}  // namespace asdfft

@@ -1,0 +1,41 @@
+// Host-side entry points of the HIP kernels (compiled in stockham.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fft_kernel_args.h"
+
+namespace asdfft {
+
+// Kernel variants (mapping / prologue / epilogue combinations that are compiled).
+enum Variant : int {
+    kVarRowsC2C = 0,
+    kVarColsC2C = 1,
+    kVarR2CEven = 2,
+    kVarC2REven = 3,
+    kVarR2COdd = 4,
+    kVarC2ROdd = 5,
+    kVarColsTwiddle = 6,
+    kVarRowsTranspose = 7,
+    kVarDirect = 8,  // dft_direct_kernel
+};
+
+struct LaunchShape {
+    int variant;
+    int emax;          // registers per thread (complex points): 16 or 32
+    int block;         // threads per workgroup
+    size_t lds_bytes;  // dynamic LDS
+    int64_t grid;      // workgroups
+};
+
+hipError_t launch_stockham(const LaunchShape& s, const FftArgs& args, hipStream_t stream);
+hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStream_t stream);
+
+}  // namespace asdfft
+
+// Compiled (variant, EMAX) instances; each lives in its own translation unit
+// (stockham_inst.hip built with -DASDFFT_VARIANT=v -DASDFFT_EMAX=e).
+#define ASDFFT_FOR_EACH_INSTANCE(X) \
+    X(0, 16) X(1, 16) X(2, 16) X(3, 16) X(4, 16) X(5, 16) X(6, 16) X(7, 16) \
+    X(0, 32) X(1, 32) X(2, 32) X(3, 32) X(4, 32) X(5, 32) X(6, 32) X(7, 32)

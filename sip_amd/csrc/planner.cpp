@@ -1,0 +1,614 @@
+#include "planner.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <map>
+#include <sstream>
+
+namespace asdfft {
+
+namespace {
+
+constexpr int kPrimes[] = {2, 3, 5, 7, 11, 13, 17, 19};
+constexpr int kNumPrimes = 8;
+constexpr size_t kLdsLimit = 160 * 1024;      // gfx950 LDS per workgroup
+constexpr size_t kLdsPreferred = 64 * 1024;   // keep >= 2 workgroups per CU when batching rows
+
+struct RadixInfo {
+    int radix;
+    int exps[kNumPrimes];
+    double cost;  // relative per-point butterfly cost, tie-breaker only
+};
+
+const RadixInfo kRadices[] = {
+    {16, {4, 0, 0, 0, 0, 0, 0, 0}, 1.25}, {12, {2, 1, 0, 0, 0, 0, 0, 0}, 1.6}, {10, {1, 0, 1, 0, 0, 0, 0, 0}, 1.7},
+    {9, {0, 2, 0, 0, 0, 0, 0, 0}, 1.6},   {8, {3, 0, 0, 0, 0, 0, 0, 0}, 1.1},  {6, {1, 1, 0, 0, 0, 0, 0, 0}, 1.4},
+    {4, {2, 0, 0, 0, 0, 0, 0, 0}, 1.0},   {2, {1, 0, 0, 0, 0, 0, 0, 0}, 1.0},  {3, {0, 1, 0, 0, 0, 0, 0, 0}, 1.3},
+    {5, {0, 0, 1, 0, 0, 0, 0, 0}, 1.6},   {7, {0, 0, 0, 1, 0, 0, 0, 0}, 2.2},  {11, {0, 0, 0, 0, 1, 0, 0, 0}, 3.0},
+    {13, {0, 0, 0, 0, 0, 1, 0, 0}, 3.4},  {17, {0, 0, 0, 0, 0, 0, 1, 0}, 4.3}, {19, {0, 0, 0, 0, 0, 0, 0, 1}, 4.8},
+};
+
+struct Best {
+    int passes = 1 << 30;
+    int maxr = 0;
+    double cost = 0;
+    std::vector<int> radices;
+};
+
+bool better(const Best& a, const Best& b)
+{
+    if (a.passes != b.passes) return a.passes < b.passes;
+    const bool a16 = a.maxr <= 16, b16 = b.maxr <= 16;
+    if (a16 != b16) return a16;
+    return a.cost < b.cost - 1e-9;
+}
+
+Best search(std::vector<int>& e, std::map<std::vector<int>, Best>& memo)
+{
+    bool done = true;
+    for (int v : e) done = done && (v == 0);
+    if (done) {
+        Best b;
+        b.passes = 0;
+        return b;
+    }
+    auto it = memo.find(e);
+    if (it != memo.end()) return it->second;
+    Best best;
+    for (const RadixInfo& r : kRadices) {
+        bool ok = true;
+        for (int i = 0; i < kNumPrimes; i++) ok = ok && (r.exps[i] <= e[i]);
+        if (!ok) continue;
+        for (int i = 0; i < kNumPrimes; i++) e[i] -= r.exps[i];
+        Best sub = search(e, memo);
+        for (int i = 0; i < kNumPrimes; i++) e[i] += r.exps[i];
+        if (sub.passes >= (1 << 30)) continue;
+        Best cand = sub;
+        cand.passes += 1;
+        cand.maxr = std::max(cand.maxr, r.radix);
+        cand.cost += r.cost;
+        cand.radices.push_back(r.radix);
+        if (better(cand, best)) best = cand;
+    }
+    memo[e] = best;
+    return best;
+}
+
+int64_t pad16(int64_t x) { return x + (x >> 4); }
+
+uint32_t magic_for(int span)
+{
+    if (span <= 1) return 0;
+    return (uint32_t)((((uint64_t)1) << 32) / (uint64_t)span + 1);
+}
+
+std::string radix_key(int64_t n, const std::vector<int>& r)
+{
+    std::ostringstream os;
+    os << "stk:" << n;
+    for (int v : r) os << ":" << v;
+    return os.str();
+}
+
+struct Config {
+    std::vector<int> radices;
+    int emax = 0;
+    int threads = 0;
+    int tpb = 1;
+    int64_t tstride = 0;  // rows mapping
+    int64_t plane = 0;
+    size_t lds = 0;
+};
+
+// Rows mapping: lanes along a transform; several short transforms per workgroup.
+// want_tpb > 0 asks for that many transforms per workgroup (transposed stores need
+// >= 16 adjacent transforms for 128-byte runs) and may use the whole LDS.
+bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, Config* c)
+{
+    std::vector<int> radices = choose_radices(n);
+    if (radices.empty()) return false;
+    bool found = false;
+    Config best;
+    for (int emax : {16, 32}) {
+        const int T = threads_for(n, radices, emax);
+        if (T == 0) continue;
+        const int maxb = max_block_for(emax);
+        const int target = want_tpb > 0 ? want_tpb : std::max(1, 256 / T);
+        int tpb = std::min(target, maxb / T);
+        tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, count));
+        const bool need_lds = radices.size() > 1 || epi_lds;
+        const int64_t tstride = pad16(n) + 1;
+        const size_t cap = (want_tpb > 0) ? kLdsLimit : kLdsPreferred;
+        while (tpb > 1 && need_lds && (size_t)(8 * tpb * tstride) > cap) tpb--;
+        const size_t lds = need_lds ? (size_t)(8 * tpb * tstride) : 0;
+        if (lds > kLdsLimit) continue;
+        Config cand;
+        cand.radices = radices;
+        cand.emax = emax;
+        cand.threads = T;
+        cand.tpb = tpb;
+        cand.tstride = tstride;
+        cand.plane = tpb * tstride;
+        cand.lds = lds;
+        if (!found || (want_tpb > 0 && cand.tpb > best.tpb)) best = cand;
+        found = true;
+        if (want_tpb <= 0) break;  // first feasible register budget (EMAX 16 preferred)
+    }
+    if (found) *c = best;
+    return found;
+}
+
+// Column mapping: lanes across adjacent transforms (columns) for coalesced strided access.
+bool cols_config(int64_t n, int64_t count, Config* c)
+{
+    std::vector<int> radices = choose_radices(n);
+    if (radices.empty()) return false;
+    bool found = false;
+    for (int tpb : {16, 8, 4, 2, 1}) {
+        for (int emax : {16, 32}) {
+            const int T = threads_for(n, radices, emax);
+            if (T == 0 || tpb * T > max_block_for(emax)) continue;
+            const int64_t plane = pad16(n * tpb) + 1;
+            const bool need_lds = radices.size() > 1;
+            const size_t lds = need_lds ? (size_t)(8 * plane) : 0;
+            if (lds > kLdsLimit) continue;
+            c->radices = radices;
+            c->emax = emax;
+            c->threads = T;
+            c->tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, count));
+            c->tstride = 0;
+            c->plane = plane;
+            c->lds = lds;
+            found = true;
+            break;
+        }
+        if (found) break;
+    }
+    return found;
+}
+
+void fill_passes(const Config& c, FftArgs* a, std::vector<int>* offsets)
+{
+    a->passes = (int)c.radices.size();
+    int span = 1;
+    for (size_t p = 0; p < c.radices.size(); p++) {
+        a->pass[p].radix = c.radices[p];
+        a->pass[p].span = span;
+        a->pass[p].magic = magic_for(span);
+        a->pass[p].tw_offset = (*offsets)[p];
+        span *= c.radices[p];
+    }
+}
+
+// A Stockham launch.  Addressing: transform t at (t % cols)*inner + (t / cols)*outer.
+struct StepIo {
+    int64_t count = 0;
+    int64_t cols = 1;
+    int64_t in_inner = 0, in_outer = 0, in_stride = 1;
+    int64_t out_inner = 0, out_outer = 0, out_stride = 1;
+};
+
+bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, int conj_in, int conj_out,
+                   int want_tpb, Step* st, std::string* why)
+{
+    Config c;
+    const bool epi_lds = (variant == kVarR2CEven || variant == kVarRowsTranspose);
+    const bool ok = cols_mapping ? cols_config(n, io.count, &c) : rows_config(n, io.count, epi_lds, want_tpb, &c);
+    if (!ok) {
+        *why = "no single-workgroup schedule for length " + std::to_string(n);
+        return false;
+    }
+    std::vector<int> offsets;
+    std::vector<float> tab = stockham_table((int)n, c.radices, &offsets);
+    TablePtr t = upload_table(radix_key(n, c.radices), tab);
+    st->tables_ready = (t != nullptr);
+    if (t) st->tables.push_back(t);
+    FftArgs& a = st->args;
+    a = FftArgs{};
+    a.tw = t ? t->dev : nullptr;
+    a.n = (int)n;
+    a.threads = c.threads;
+    a.tpb = c.tpb;
+    a.lds_tstride = (int)c.tstride;
+    a.lds_plane = (int)c.plane;
+    a.conj_in = conj_in;
+    a.conj_out = conj_out;
+    a.count = io.count;
+    a.cols = io.cols;
+    a.in_inner = io.in_inner;
+    a.in_outer = io.in_outer;
+    a.in_stride = io.in_stride;
+    a.out_inner = io.out_inner;
+    a.out_outer = io.out_outer;
+    a.out_stride = io.out_stride;
+    fill_passes(c, &a, &offsets);
+    st->direct = false;
+    st->shape.variant = variant;
+    st->shape.emax = c.emax;
+    st->shape.block = c.tpb * c.threads;
+    st->shape.lds_bytes = c.lds;
+    st->shape.grid = (io.count + c.tpb - 1) / c.tpb;
+    std::ostringstream os;
+    os << "stockham variant=" << variant << " n=" << n << " count=" << io.count << " radices=";
+    for (size_t i = 0; i < c.radices.size(); i++) os << (i ? "x" : "") << c.radices[i];
+    os << " emax=" << c.emax << " T=" << c.threads << " tpb=" << c.tpb << " block=" << st->shape.block
+       << " lds=" << c.lds << " grid=" << st->shape.grid;
+    st->describe = os.str();
+    return true;
+}
+
+bool attach_aux(Step* st, int64_t nreal)
+{
+    const int64_t m = nreal / 2;
+    TablePtr t = upload_table("aux:" + std::to_string(nreal), roots_table(nreal, m + 1));
+    if (t) st->tables.push_back(t);
+    st->tables_ready = st->tables_ready && (t != nullptr);
+    st->args.tw_aux = t ? t->dev : nullptr;
+    return true;
+}
+
+bool direct_step(int64_t n, int prologue, int epilogue, int64_t count, int64_t in_dist, int64_t out_dist, int conj_in,
+                 int conj_out, Step* st, std::string* why)
+{
+    if (n > kMaxDirect) {
+        *why = "length " + std::to_string(n) + " has prime factors above 19 and exceeds the direct-DFT limit";
+        return false;
+    }
+    TablePtr t = upload_table("roots:" + std::to_string(n), roots_table(n, n));
+    st->tables_ready = (t != nullptr);
+    if (t) st->tables.push_back(t);
+    DftArgs& d = st->dargs;
+    d = DftArgs{};
+    d.roots = t ? t->dev : nullptr;
+    d.n = (int)n;
+    int tpb = (int)std::max<int64_t>(1, std::min<int64_t>(count, std::max<int64_t>(1, 4096 / n)));
+    d.tpb = tpb;
+    d.conj_in = conj_in;
+    d.conj_out = conj_out;
+    d.prologue = prologue;
+    d.epilogue = epilogue;
+    d.count = count;
+    d.in_dist = in_dist;
+    d.out_dist = out_dist;
+    st->direct = true;
+    st->shape.variant = kVarDirect;
+    st->shape.emax = 0;
+    st->shape.block = 256;
+    st->shape.lds_bytes = (size_t)tpb * (size_t)n * 8;
+    st->shape.grid = (count + tpb - 1) / tpb;
+    std::ostringstream os;
+    os << "direct n=" << n << " count=" << count << " pro=" << prologue << " epi=" << epilogue << " tpb=" << tpb;
+    st->describe = os.str();
+    return true;
+}
+
+bool smooth_single(int64_t n) { return n > 1 && n <= kMaxSingleKernel && !choose_radices(n).empty(); }
+
+// ---- 1D building blocks -------------------------------------------------------------
+// Contiguous complex rows of length n: count rows, in/out row distance n.
+bool c2c_rows(int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPlan* bp, std::string* why)
+{
+    Step st;
+    st.src = src;
+    st.dst = dst;
+    st.bytes = 16.0 * (double)n * (double)count;
+    if (n == 1 || !smooth_single(n)) {
+        if (n > kMaxSingleKernel && !choose_radices(n).empty()) {
+            *why = "large length handled by four-step";
+            return false;
+        }
+        if (!direct_step(n, kProComplex, kEpiComplex, count, n, n, inverse, inverse, &st, why)) return false;
+    } else {
+        StepIo io;
+        io.count = count;
+        io.cols = 1;
+        io.in_outer = n;
+        io.out_outer = n;
+        if (!stockham_step(n, kVarRowsC2C, false, io, inverse, inverse, 0, &st, why)) return false;
+    }
+    bp->steps.push_back(std::move(st));
+    return true;
+}
+
+// Strided complex transforms of length n along an axis with element stride `stride`;
+// `cols` adjacent transforms per outer group, groups `outer` elements apart.
+bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t stride, bool inverse, int src, int dst,
+              BuiltPlan* bp, std::string* why)
+{
+    if (n == 1) {
+        if (src == dst) return true;  // identity along this axis
+        // length-1 axis: the transforms are the elements themselves, contiguous
+        Step st;
+        st.src = src;
+        st.dst = dst;
+        st.bytes = 16.0 * (double)count;
+        if (!direct_step(1, kProComplex, kEpiComplex, count, 1, 1, inverse, inverse, &st, why)) return false;
+        bp->steps.push_back(std::move(st));
+        return true;
+    }
+    if (!smooth_single(n)) {
+        *why = "axis length " + std::to_string(n) + " unsupported for a strided pass";
+        return false;
+    }
+    Step st;
+    st.src = src;
+    st.dst = dst;
+    st.bytes = 16.0 * (double)n * (double)count;
+    StepIo io;
+    io.count = count;
+    io.cols = cols;
+    io.in_inner = 1;
+    io.in_outer = outer;
+    io.in_stride = stride;
+    io.out_inner = 1;
+    io.out_outer = outer;
+    io.out_stride = stride;
+    if (!stockham_step(n, kVarColsC2C, true, io, inverse, inverse, 0, &st, why)) return false;
+    bp->steps.push_back(std::move(st));
+    return true;
+}
+
+// Large complex length: n = n1 * n2, pass A = length-n2 columns (stride n1) with the
+// W_n^{n1 k2} twiddle folded into its store, pass B = length-n1 rows stored transposed.
+bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size_t work_offset, BuiltPlan* bp,
+                  std::string* why)
+{
+    int64_t best1 = 0;
+    double best_score = -1e30;
+    for (int64_t n1 = 2; n1 <= 1024 && n1 < n; n1++) {
+        if (n % n1) continue;
+        const int64_t n2 = n / n1;
+        if (n2 > 4096 || !smooth_single(n1) || !smooth_single(n2)) continue;
+        Config ca, cb;
+        if (!cols_config(n2, count * n1, &ca)) continue;
+        if (!rows_config(n1, count * n2, true, 16, &cb)) continue;
+        const double score = 4.0 * std::min(ca.tpb, cb.tpb) - fabs(log2((double)n1) - log2((double)n2));
+        if (score > best_score) {
+            best_score = score;
+            best1 = n1;
+        }
+    }
+    if (best1 == 0) {
+        *why = "length " + std::to_string(n) + " has no supported two-pass split";
+        return false;
+    }
+    const int64_t n1 = best1, n2 = n / n1;
+    int shift = 0;
+    while (((int64_t)1 << shift) * ((int64_t)1 << shift) < n) shift++;
+    TablePtr lo = upload_table("biglo:" + std::to_string(n) + ":" + std::to_string(shift),
+                               roots_table(n, (int64_t)1 << shift));
+    const int64_t nhi = (n + ((int64_t)1 << shift) - 1) >> shift;
+    TablePtr hi = upload_table("bighi:" + std::to_string(n) + ":" + std::to_string(shift),
+                               roots_table(n, nhi, (int64_t)1 << shift));
+
+    Step a;
+    a.src = src;
+    a.dst = kBufWork;
+    a.dst_offset = work_offset;
+    a.bytes = 16.0 * (double)n * (double)count;
+    StepIo ia;
+    ia.count = count * n1;
+    ia.cols = n1;
+    ia.in_inner = 1;
+    ia.in_outer = n;
+    ia.in_stride = n1;
+    ia.out_inner = 1;
+    ia.out_outer = n;
+    ia.out_stride = n1;
+    if (!stockham_step(n2, kVarColsTwiddle, true, ia, inverse, 0, 0, &a, why)) return false;
+    a.args.tw_big_lo = lo ? lo->dev : nullptr;
+    a.args.tw_big_hi = hi ? hi->dev : nullptr;
+    a.args.big_shift = shift;
+    if (lo) a.tables.push_back(lo);
+    if (hi) a.tables.push_back(hi);
+    a.tables_ready = a.tables_ready && lo && hi;
+    a.describe += " (four-step A n=" + std::to_string(n) + ")";
+
+    Step b;
+    b.src = kBufWork;
+    b.src_offset = work_offset;
+    b.dst = dst;
+    b.bytes = 16.0 * (double)n * (double)count;
+    StepIo ib;
+    ib.count = count * n2;
+    ib.cols = n2;
+    ib.in_inner = n1;
+    ib.in_outer = n;
+    ib.in_stride = 1;
+    ib.out_inner = 1;
+    ib.out_outer = n;
+    ib.out_stride = n2;
+    if (!stockham_step(n1, kVarRowsTranspose, false, ib, 0, inverse, 16, &b, why)) return false;
+    b.describe += " (four-step B)";
+    bp->steps.push_back(std::move(a));
+    bp->steps.push_back(std::move(b));
+    return true;
+}
+
+// Real rows: R2C (in real [count][n] -> out [count][n/2+1]) or C2R (reverse).
+bool real_rows(Kind kind, int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPlan* bp, std::string* why)
+{
+    const int64_t h = n / 2 + 1;
+    Step st;
+    st.src = src;
+    st.dst = dst;
+    st.bytes = (4.0 * (double)n + 8.0 * (double)h) * (double)count;
+    const bool even = (n % 2) == 0;
+    if (even && n >= 4 && smooth_single(n / 2)) {
+        const int64_t m = n / 2;
+        StepIo io;
+        io.count = count;
+        io.cols = 1;
+        if (kind == Kind::R2C) {
+            io.in_outer = m;       // float2 units
+            io.out_outer = m + 1;
+            if (!stockham_step(m, kVarR2CEven, false, io, 0, inverse, 0, &st, why)) return false;
+        } else {
+            io.in_outer = m + 1;
+            io.out_outer = m;
+            if (!stockham_step(m, kVarC2REven, false, io, inverse ? 0 : 1, 1, 0, &st, why)) return false;
+        }
+        attach_aux(&st, n);
+    } else if (smooth_single(n)) {
+        StepIo io;
+        io.count = count;
+        io.cols = 1;
+        if (kind == Kind::R2C) {
+            io.in_outer = n;  // floats
+            io.out_outer = h;
+            if (!stockham_step(n, kVarR2COdd, false, io, 0, inverse, 0, &st, why)) return false;
+        } else {
+            io.in_outer = h;
+            io.out_outer = n;  // floats
+            if (!stockham_step(n, kVarC2ROdd, false, io, inverse, 0, 0, &st, why)) return false;
+        }
+    } else {
+        if (kind == Kind::R2C) {
+            if (!direct_step(n, kProReal, kEpiHalf, count, n, h, 0, inverse, &st, why)) return false;
+        } else {
+            if (!direct_step(n, kProHerm, kEpiReal, count, h, n, inverse, 0, &st, why)) return false;
+        }
+    }
+    bp->steps.push_back(std::move(st));
+    return true;
+}
+
+}  // namespace
+
+int max_block_for(int emax) { return emax <= 16 ? 256 : 512; }
+
+std::vector<int> choose_radices(int64_t n)
+{
+    if (n <= 1) return {};
+    std::vector<int> e(kNumPrimes, 0);
+    int64_t m = n;
+    for (int i = 0; i < kNumPrimes; i++) {
+        while (m % kPrimes[i] == 0) {
+            m /= kPrimes[i];
+            e[i]++;
+        }
+    }
+    if (m != 1) return {};
+    std::map<std::vector<int>, Best> memo;
+    Best b = search(e, memo);
+    if (b.passes >= (1 << 30)) return {};
+    std::vector<int> r = b.radices;
+    std::sort(r.begin(), r.end(), [](int x, int y) { return x > y; });
+    if ((int)r.size() > kMaxPasses) return {};
+    return r;
+}
+
+int threads_for(int64_t n, const std::vector<int>& radices, int emax)
+{
+    int64_t t = 0;
+    for (int r : radices) {
+        if (r > emax) return 0;
+        const int64_t b = emax / r;
+        const int64_t need = (n / r + b - 1) / b;
+        t = std::max(t, need);
+    }
+    if (t <= 0 || t > max_block_for(emax)) return 0;
+    return (int)t;
+}
+
+std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
+{
+    out->steps.clear();
+    out->workspace = 0;
+    std::string why;
+    const bool inv = spec.inverse;
+    const int64_t B = spec.batch;
+    const size_t nd = spec.dims.size();
+    if (nd < 1 || nd > 3) return "unsupported rank";
+
+    if (nd == 1) {
+        const int64_t n = spec.dims[0];
+        if (spec.vertical) {
+            // tensor (n, batch): transform along axis 0, `batch` adjacent columns
+            if (spec.kind != Kind::C2C) return "VERTICAL layout is supported for C2C only";
+            if (B == 1) {
+                if (!c2c_rows(n, 1, inv, kBufIn, kBufOut, out, &why)) return why;
+                return "";
+            }
+            if (!c2c_cols(n, B, B, 0, B, inv, kBufIn, kBufOut, out, &why)) return why;
+            return "";
+        }
+        if (spec.kind == Kind::C2C) {
+            if (n > kMaxSingleKernel && !choose_radices(n).empty()) {
+                out->workspace = (size_t)(B * n * 8);
+                if (!c2c_fourstep(n, B, inv, kBufIn, kBufOut, 0, out, &why)) return why;
+                return "";
+            }
+            if (!c2c_rows(n, B, inv, kBufIn, kBufOut, out, &why)) return why;
+            return "";
+        }
+        if (!real_rows(spec.kind, n, B, inv, kBufIn, kBufOut, out, &why)) return why;
+        return "";
+    }
+
+    // multi-dimensional: dims = (X, Y) or (X, Y, Z); last axis contiguous
+    const int64_t L = spec.dims[nd - 1];
+    const int64_t W = (spec.kind == Kind::C2C) ? L : L / 2 + 1;  // complex width of the last axis
+    int64_t lead = 1;
+    for (size_t d = 0; d + 1 < nd; d++) lead *= spec.dims[d];
+    const int64_t rows = B * lead;
+
+    // column passes over the complex array [B][dims[0..nd-2]][W], axis d (d < nd-1)
+    auto col_pass = [&](size_t d, int src, int dst) -> bool {
+        const int64_t len = spec.dims[d];
+        int64_t inner = W;  // elements between consecutive points along axis d
+        for (size_t e = d + 1; e + 1 < nd; e++) inner *= spec.dims[e];
+        const int64_t outer = inner * len;
+        const int64_t count = B * lead / len * W;
+        if (len == 1) {
+            if (src == dst) return true;
+        }
+        return c2c_cols(len, count, inner, outer, inner, inv, src, dst, out, &why);
+    };
+
+    if (spec.kind == Kind::C2C) {
+        if (L > kMaxSingleKernel) return "last-axis length above " + std::to_string(kMaxSingleKernel);
+        if (!c2c_rows(L, rows, inv, kBufIn, kBufOut, out, &why)) return why;
+        for (size_t d = nd - 1; d-- > 0;) {
+            if (!col_pass(d, kBufOut, kBufOut)) return why;
+        }
+        return "";
+    }
+    if (L / 2 > kMaxSingleKernel) return "last-axis length above " + std::to_string(2 * kMaxSingleKernel);
+    if (spec.kind == Kind::R2C) {
+        if (!real_rows(Kind::R2C, L, rows, inv, kBufIn, kBufOut, out, &why)) return why;
+        for (size_t d = nd - 1; d-- > 0;) {
+            if (!col_pass(d, kBufOut, kBufOut)) return why;
+        }
+        return "";
+    }
+    // C2R: complex column passes into the workspace, then C2R rows into the output
+    out->workspace = (size_t)(B * lead * W * 8);
+    bool first = true;
+    for (size_t d = nd - 1; d-- > 0;) {
+        const int src = first ? kBufIn : kBufWork;
+        if (spec.dims[d] == 1 && first) {
+            // identity axis: still has to move the data into the workspace
+            if (!c2c_rows(1, B * lead * W, inv, src, kBufWork, out, &why)) return why;
+        } else if (!col_pass(d, src, kBufWork)) {
+            return why;
+        }
+        first = false;
+    }
+    if (!real_rows(Kind::C2R, L, rows, inv, kBufWork, kBufOut, out, &why)) return why;
+    return "";
+}
+
+std::string describe_plan(const BuiltPlan& plan)
+{
+    std::ostringstream os;
+    os << "plan: " << plan.steps.size() << " step(s), workspace=" << plan.workspace << " B\n";
+    const char* names[] = {"in", "out", "work"};
+    for (const Step& s : plan.steps) {
+        os << "  [" << names[s.src] << "->" << names[s.dst] << "] " << s.describe << "\n";
+    }
+    return os.str();
+}
+
+}  // namespace asdfft

@@ -1,0 +1,569 @@
+#pragma once
+// Device implementation of the batched Stockham FFT kernels for gfx950 (MI355X).
+//
+// One workgroup owns `tpb` transforms.  Each thread keeps up to EMAX complex
+// points in VGPRs; a pass of radix R performs EMAX/R butterflies per thread,
+// reading its inputs at stride n/R and writing them to the autosort position
+// (j - k)*R + k + r*L (k = j mod L).  The first pass reads HBM directly (lane
+// -contiguous float2 streams), the last pass writes HBM directly; the passes in
+// between exchange through LDS (separate re/im planes, one pad word per 16).
+// Inverse transforms use conj(DFT(conj(x))): only forward twiddles exist.
+//
+// Capability it replaces: the reference's per-row Stockham device kernels
+// (ops/fft/c2c/fft_regbase/fft_regbase/kernel/arch35/fft_c2c_regbase.asc:20-64,
+// ops/fft/common/mixed/kernel/mixed.asc:15-31, real boundary pre/post twiddles
+// ops/fft/real/fft_regbase/fft_regbase/kernel/arch35/vendor/regbase_real_boundary_impl.h:25-80),
+// written here from the DFT definition for wave64 / LDS.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_kernel_args.h"
+#include "kernels.h"
+#include "radix_consts.h"
+
+namespace asdfft {
+
+// ------------------------------------------------------------------ helpers
+template <int V>
+struct IC {
+    static constexpr int value = V;
+};
+
+template <int Start, int End, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (Start < End) {
+        f(IC<Start>{});
+        static_for<Start + 1, End>(f);
+    }
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }  // a * (-i)
+__device__ __forceinline__ float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }  // a * (+i)
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// a * e^{-2 pi i M / R} with the angle known at compile time
+template <int R, int M>
+__device__ __forceinline__ float2 twc(float2 x)
+{
+    constexpr int m = M % R;
+    if constexpr (m == 0) {
+        return x;
+    } else if constexpr (4 * m == R) {
+        return mul_mi(x);
+    } else if constexpr (2 * m == R) {
+        return make_float2(-x.x, -x.y);
+    } else if constexpr (4 * m == 3 * R) {
+        return mul_pi(x);
+    } else if constexpr (8 * m == R) {
+        constexpr float h = (float)RootTable<R>::c[m];
+        return make_float2(h * (x.x + x.y), h * (x.y - x.x));
+    } else if constexpr (8 * m == 3 * R) {
+        constexpr float h = (float)RootTable<R>::s[m];
+        return make_float2(h * (x.y - x.x), -h * (x.x + x.y));
+    } else {
+        constexpr float c = (float)RootTable<R>::c[m];
+        constexpr float s = (float)RootTable<R>::s[m];
+        return make_float2(x.x * c + x.y * s, x.y * c - x.x * s);
+    }
+}
+
+// ------------------------------------------------------------------ codelets
+// Forward DFT of R points held in v[0..R-1], natural order in and out.
+template <int R>
+struct Dft;
+
+template <>
+struct Dft<1> {
+    static __device__ __forceinline__ void run(float2*) {}
+};
+
+template <>
+struct Dft<2> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 a = cadd(v[0], v[1]);
+        v[1] = csub(v[0], v[1]);
+        v[0] = a;
+    }
+};
+
+template <>
+struct Dft<3> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr float s3 = (float)RootTable<3>::s[1];  // sin(2 pi / 3)
+        float2 a = cadd(v[1], v[2]);
+        float2 b = csub(v[1], v[2]);
+        float2 t = make_float2(v[0].x - 0.5f * a.x, v[0].y - 0.5f * a.y);
+        float2 s = make_float2(b.x * s3, b.y * s3);
+        v[0] = cadd(v[0], a);
+        v[1] = make_float2(t.x + s.y, t.y - s.x);
+        v[2] = make_float2(t.x - s.y, t.y + s.x);
+    }
+};
+
+template <>
+struct Dft<4> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 t0 = cadd(v[0], v[2]);
+        float2 t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]);
+        float2 t3 = mul_mi(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+// Odd prime P: pairs (n, P-n) share cosines; X_k = v0 + sum a_n cos - i sum b_n sin.
+template <int P>
+struct DftOdd {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr int H = (P - 1) / 2;
+        float2 a[H + 1], b[H + 1];
+        const float2 x0 = v[0];
+        float2 sum = x0;
+        static_for<1, H + 1>([&](auto I) __attribute__((always_inline)) {
+            constexpr int n = decltype(I)::value;
+            a[n] = cadd(v[n], v[P - n]);
+            b[n] = csub(v[n], v[P - n]);
+            sum = cadd(sum, a[n]);
+        });
+        static_for<1, H + 1>([&](auto K) __attribute__((always_inline)) {
+            constexpr int k = decltype(K)::value;
+            float2 re = x0;
+            float2 im = make_float2(0.0f, 0.0f);
+            static_for<1, H + 1>([&](auto I) __attribute__((always_inline)) {
+                constexpr int n = decltype(I)::value;
+                constexpr int m = (n * k) % P;
+                constexpr float c = (float)RootTable<P>::c[m];
+                constexpr float s = (float)RootTable<P>::s[m];
+                re.x += a[n].x * c;
+                re.y += a[n].y * c;
+                im.x += b[n].x * s;
+                im.y += b[n].y * s;
+            });
+            v[k] = make_float2(re.x + im.y, re.y - im.x);
+            v[P - k] = make_float2(re.x - im.y, re.y + im.x);
+        });
+        v[0] = sum;
+    }
+};
+
+template <> struct Dft<5> : DftOdd<5> {};
+template <> struct Dft<7> : DftOdd<7> {};
+template <> struct Dft<11> : DftOdd<11> {};
+template <> struct Dft<13> : DftOdd<13> {};
+template <> struct Dft<17> : DftOdd<17> {};
+template <> struct Dft<19> : DftOdd<19> {};
+
+// Composite R = A*B, decimation in time: n = A*n2 + n1, k = k1 + B*k2.
+template <int A, int B>
+struct DftCT {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr int R = A * B;
+        float2 y[R];
+        static_for<0, A>([&](auto I1) __attribute__((always_inline)) {
+            constexpr int n1 = decltype(I1)::value;
+            float2 t[B];
+            static_for<0, B>([&](auto I2) __attribute__((always_inline)) {
+                constexpr int n2 = decltype(I2)::value;
+                t[n2] = v[A * n2 + n1];
+            });
+            Dft<B>::run(t);
+            static_for<0, B>([&](auto K1) __attribute__((always_inline)) {
+                constexpr int k1 = decltype(K1)::value;
+                y[n1 * B + k1] = twc<R, n1 * k1>(t[k1]);
+            });
+        });
+        static_for<0, B>([&](auto K1) __attribute__((always_inline)) {
+            constexpr int k1 = decltype(K1)::value;
+            float2 t[A];
+            static_for<0, A>([&](auto I1) __attribute__((always_inline)) {
+                constexpr int n1 = decltype(I1)::value;
+                t[n1] = y[n1 * B + k1];
+            });
+            Dft<A>::run(t);
+            static_for<0, A>([&](auto K2) __attribute__((always_inline)) {
+                constexpr int k2 = decltype(K2)::value;
+                v[k1 + B * k2] = t[k2];
+            });
+        });
+    }
+};
+
+template <> struct Dft<6> : DftCT<2, 3> {};
+template <> struct Dft<8> : DftCT<2, 4> {};
+template <> struct Dft<9> : DftCT<3, 3> {};
+template <> struct Dft<10> : DftCT<2, 5> {};
+template <> struct Dft<12> : DftCT<4, 3> {};
+template <> struct Dft<16> : DftCT<4, 4> {};
+
+// ------------------------------------------------------------------ LDS layout
+__device__ __forceinline__ int pad16(int x) { return x + (x >> 4); }
+
+template <int MAP>
+__device__ __forceinline__ int lds_at(const FftArgs& a, int tt, int i)
+{
+    if constexpr (MAP == kMapRows) {
+        return tt * a.lds_tstride + pad16(i);
+    } else {
+        return pad16(i * a.tpb + tt);
+    }
+}
+
+__device__ __forceinline__ float2 ldg2(const float* p, int64_t idx)
+{
+    return reinterpret_cast<const float2*>(p)[idx];
+}
+
+// ------------------------------------------------------------------ prologue / epilogue
+// Per-thread view of one transform: base pointers plus 32-bit element strides.
+struct Io {
+    const void* in;   // first element of this transform's input
+    void* out;        // first element of this transform's output
+    int64_t col;      // column index (four-step twiddle)
+};
+
+template <int MAP>
+__device__ __forceinline__ int in_stride(const FftArgs& a)
+{
+    if constexpr (MAP == kMapRows) return 1;
+    return (int)a.in_stride;
+}
+
+template <int MAP>
+__device__ __forceinline__ int out_stride(const FftArgs& a)
+{
+    if constexpr (MAP == kMapRows) return 1;
+    return (int)a.out_stride;
+}
+
+template <int MAP, int PRO>
+__device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int i)
+{
+    const float2* in2 = reinterpret_cast<const float2*>(io.in);
+    const int s = in_stride<MAP>(a);
+    if constexpr (PRO == kProComplex) {
+        float2 x = in2[i * s];
+        if (a.conj_in) x.y = -x.y;
+        return x;
+    } else if constexpr (PRO == kProReal) {
+        return make_float2(reinterpret_cast<const float*>(io.in)[i * s], 0.0f);
+    } else if constexpr (PRO == kProHerm) {
+        const int h = a.n >> 1;
+        float2 x;
+        if (i <= h) {
+            x = in2[i * s];
+            if (i == 0 || 2 * i == a.n) x.y = 0.0f;
+        } else {
+            x = in2[(a.n - i) * s];
+            x.y = -x.y;
+        }
+        if (a.conj_in) x.y = -x.y;
+        return x;
+    } else {  // kProC2REven: complex length M = a.n, input holds M + 1 bins
+        const int m = a.n;
+        float2 xa = in2[i * s];
+        float2 xb = in2[(m - i) * s];
+        if (a.conj_in) {
+            xa.y = -xa.y;
+            xb.y = -xb.y;
+        }
+        if (i == 0) {
+            xa.y = 0.0f;
+            xb.y = 0.0f;
+        }
+        const float2 sm = make_float2(xa.x + xb.x, xa.y - xb.y);  // xa + conj(xb)
+        const float2 d = make_float2(xa.x - xb.x, xa.y + xb.y);   // xa - conj(xb)
+        const float2 w = conjf2(ldg2(a.tw_aux, i));               // W_N^{-i}
+        const float2 q = cmul(w, d);
+        // Z = sm + i*q ; the forward FFT consumes conj(Z)
+        return make_float2(sm.x - q.y, -(sm.y + q.x));
+    }
+}
+
+template <int MAP, int EPI>
+__device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int i, float2 x)
+{
+    float2* out2 = reinterpret_cast<float2*>(io.out);
+    const int s = out_stride<MAP>(a);
+    if constexpr (EPI == kEpiComplex) {
+        if (a.conj_out) x.y = -x.y;
+        out2[i * s] = x;
+    } else if constexpr (EPI == kEpiHalf) {
+        if (2 * i <= a.n) {
+            if (a.conj_out) x.y = -x.y;
+            out2[i * s] = x;
+        }
+    } else if constexpr (EPI == kEpiReal) {
+        reinterpret_cast<float*>(io.out)[i * s] = x.x;
+    } else if constexpr (EPI == kEpiTwiddle) {
+        const uint32_t m = (uint32_t)io.col * (uint32_t)i;  // col * i < big length <= 2^31
+        const uint32_t mask = (1u << a.big_shift) - 1u;
+        const float2 w = cmul(ldg2(a.tw_big_hi, m >> a.big_shift), ldg2(a.tw_big_lo, m & mask));
+        x = cmul(x, w);
+        if (a.conj_out) x.y = -x.y;
+        out2[i * s] = x;
+    }
+}
+
+// ------------------------------------------------------------------ one pass
+template <int EMAX, int R, int MAP, int PRO, int EPI>
+__device__ __forceinline__ void stockham_pass(const FftArgs& a, int p, float* lre, float* lim, int tt, int tid, bool live,
+                                              const Io& io)
+{
+    float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
+    constexpr int B = EMAX / R;
+    static_assert(B >= 1, "radix larger than the per-thread register budget");
+    constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose);
+    // The pass loop calls one of many radix cases; without these opaque copies the
+    // compiler hoists every case's address arithmetic out of the loop and keeps all of
+    // it live at once (hundreds of VGPRs).  Recomputing it per pass is a few VALU ops.
+    int n = a.n;
+    int T = a.threads;
+    asm volatile("" : "+s"(n), "+s"(T));
+    asm volatile("" : "+v"(tid));
+    const int nr = n / R;
+    const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
+    const bool first = (p == 0);
+    const bool last = (p == a.passes - 1);
+
+    // 1. gather
+    if (first) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (live && j < nr) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, j + r * nr);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (j < nr) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int at = lds_at<MAP>(a, tt, j + r * nr);
+                    v[b * R + r] = make_float2(lre[at], lim[at]);
+                }
+            }
+        }
+    }
+
+    // 2. twiddle + butterfly
+    const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
+    const float2* twp = reinterpret_cast<const float2*>(a.tw) + __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+        if (j < nr) {
+            if (span > 1) {
+                const int k = j - (int)__umulhi((uint32_t)j, magic) * span;
+                const float2* w = twp + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
+            }
+            Dft<R>::run(&v[b * R]);
+        }
+    }
+
+    // 3. scatter
+    if (!last || kEpiViaLds) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (j < nr) {
+                const int k = (span > 1) ? j - (int)__umulhi((uint32_t)j, magic) * span : 0;
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int at = lds_at<MAP>(a, tt, base + r * span);
+                    lre[at] = v[b * R + r].x;
+                    lim[at] = v[b * R + r].y;
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (live && j < nr) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) store_output<MAP, EPI>(a, io, j + r * nr, v[b * R + r]);
+            }
+        }
+    }
+}
+
+template <int EMAX, int R, int MAP, int PRO, int EPI>
+__device__ __forceinline__ void maybe_pass(const FftArgs& a, int p, float* lre, float* lim, int tt, int tid, bool live,
+                                           const Io& io)
+{
+    if constexpr (R <= EMAX) {
+        stockham_pass<EMAX, R, MAP, PRO, EPI>(a, p, lre, lim, tt, tid, live, io);
+    }
+    // radices above EMAX are never scheduled: the planner checks radix <= emax before launch
+}
+
+template <int EMAX>
+constexpr int max_block()
+{
+    return EMAX <= 16 ? 256 : 512;
+}
+
+// ------------------------------------------------------------------ kernel
+template <int EMAX, int MAP, int PRO, int EPI>
+__global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftArgs a)
+{
+    extern __shared__ float smem[];
+    float* lre = smem;
+    float* lim = smem + a.lds_plane;
+    const int lane = threadIdx.x;
+    int tt, tid;
+    if constexpr (MAP == kMapRows) {
+        tt = lane / a.threads;
+        tid = lane - tt * a.threads;
+    } else {
+        tt = lane % a.tpb;
+        tid = lane / a.tpb;
+    }
+    const int64_t t = (int64_t)blockIdx.x * a.tpb + tt;
+    const bool live = t < a.count;
+    Io io;
+    io.in = a.in;
+    io.out = a.out;
+    io.col = 0;
+    if (live) {
+        io.col = t % a.cols;
+        const int64_t o = t / a.cols;
+        const int64_t ib = io.col * a.in_inner + o * a.in_outer;
+        const int64_t ob = io.col * a.out_inner + o * a.out_outer;
+        if constexpr (PRO == kProReal) {
+            io.in = reinterpret_cast<const float*>(a.in) + ib;
+        } else {
+            io.in = reinterpret_cast<const float2*>(a.in) + ib;
+        }
+        if constexpr (EPI == kEpiReal) {
+            io.out = reinterpret_cast<float*>(a.out) + ob;
+        } else {
+            io.out = reinterpret_cast<float2*>(a.out) + ob;
+        }
+    }
+    for (int p = 0; p < a.passes; ++p) {
+        // pass descriptors are uniform; readfirstlane keeps the switch a scalar branch
+        switch (__builtin_amdgcn_readfirstlane(a.pass[p].radix)) {
+#define ASDFFT_CASE(RR) \
+    case RR: maybe_pass<EMAX, RR, MAP, PRO, EPI>(a, p, lre, lim, tt, tid, live, io); break;
+            ASDFFT_CASE(16)
+            ASDFFT_CASE(12)
+            ASDFFT_CASE(10)
+            ASDFFT_CASE(9)
+            ASDFFT_CASE(8)
+            ASDFFT_CASE(6)
+            ASDFFT_CASE(4)
+            ASDFFT_CASE(2)
+            ASDFFT_CASE(19)
+            ASDFFT_CASE(17)
+            ASDFFT_CASE(13)
+            ASDFFT_CASE(11)
+            ASDFFT_CASE(7)
+            ASDFFT_CASE(5)
+            ASDFFT_CASE(3)
+#undef ASDFFT_CASE
+            default:
+                break;
+        }
+    }
+
+    if constexpr (EPI == kEpiR2CEven) {
+        // LDS holds Z = FFT_M(x[2n] + i x[2n+1]) in natural order.
+        // X[k] = 1/2 [(Z_k + conj Z_{M-k}) - i W_N^k (Z_k - conj Z_{M-k})], k in [0, M]
+        if (live) {
+            const int m = a.n;
+            float2* out2 = reinterpret_cast<float2*>(io.out);
+            for (int k = tid; 2 * k <= m; k += a.threads) {
+                const int km = (k == 0) ? 0 : m - k;
+                const int at0 = lds_at<MAP>(a, tt, k);
+                const int at1 = lds_at<MAP>(a, tt, km);
+                const float2 zk = make_float2(lre[at0], lim[at0]);
+                const float2 zm = make_float2(lre[at1], lim[at1]);
+                const float2 wk = ldg2(a.tw_aux, k);
+                const float2 wm = ldg2(a.tw_aux, m - k);
+                const float2 s0 = make_float2(zk.x + zm.x, zk.y - zm.y);
+                const float2 d0 = make_float2(zk.x - zm.x, zk.y + zm.y);
+                const float2 q0 = cmul(wk, d0);
+                float2 x0 = make_float2(0.5f * (s0.x + q0.y), 0.5f * (s0.y - q0.x));
+                const float2 s1 = make_float2(zm.x + zk.x, zm.y - zk.y);
+                const float2 d1 = make_float2(zm.x - zk.x, zm.y + zk.y);
+                const float2 q1 = cmul(wm, d1);
+                float2 x1 = make_float2(0.5f * (s1.x + q1.y), 0.5f * (s1.y - q1.x));
+                if (a.conj_out) {
+                    x0.y = -x0.y;
+                    x1.y = -x1.y;
+                }
+                out2[k] = x0;
+                if (m - k != k) out2[m - k] = x1;
+            }
+        }
+    } else if constexpr (EPI == kEpiTranspose) {
+        // LDS holds tpb finished transforms; adjacent lanes take adjacent transforms so the
+        // strided destination (transform t -> column) is written in contiguous runs.
+        const int tt2 = lane % a.tpb;
+        const int step = (int)(blockDim.x / a.tpb);
+        const int64_t t2 = (int64_t)blockIdx.x * a.tpb + tt2;
+        if (t2 < a.count) {
+            const int64_t c2 = t2 % a.cols;
+            float2* ob = reinterpret_cast<float2*>(a.out) + c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
+            const int s = (int)a.out_stride;
+            for (int i = lane / a.tpb; i < a.n; i += step) {
+                const int at = lds_at<MAP>(a, tt2, i);
+                float2 x = make_float2(lre[at], lim[at]);
+                if (a.conj_out) x.y = -x.y;
+                ob[i * s] = x;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launch helper
+template <int V> struct VariantTraits;
+template <> struct VariantTraits<kVarRowsC2C> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiComplex; };
+template <> struct VariantTraits<kVarColsC2C> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiComplex; };
+template <> struct VariantTraits<kVarR2CEven> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiR2CEven; };
+template <> struct VariantTraits<kVarC2REven> { static constexpr int map = kMapRows, pro = kProC2REven, epi = kEpiComplex; };
+template <> struct VariantTraits<kVarR2COdd> { static constexpr int map = kMapRows, pro = kProReal, epi = kEpiHalf; };
+template <> struct VariantTraits<kVarC2ROdd> { static constexpr int map = kMapRows, pro = kProHerm, epi = kEpiReal; };
+template <> struct VariantTraits<kVarColsTwiddle> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiTwiddle; };
+template <> struct VariantTraits<kVarRowsTranspose> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiTranspose; };
+
+template <int EMAX, int V>
+hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
+{
+    using Tr = VariantTraits<V>;
+    auto kern = stockham_kernel<EMAX, Tr::map, Tr::pro, Tr::epi>;
+    if (s.lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)s.grid), dim3((unsigned)s.block), s.lds_bytes, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace asdfft
