@@ -133,6 +133,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--only", default=None, help="run one workload entry by name (profiling)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is a design note, "
@@ -240,6 +242,14 @@ def main():
         line["launches"] = [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()} for r in launches]
         for j in jobs:
             print(j["describe"], file=sys.stderr)
+    if a.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["calibration_copy_bytes"] = {"read": src.numel() * 4, "write": dst.numel() * 4}
+        del src, dst
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     for j in jobs:

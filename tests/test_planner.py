@@ -1,0 +1,85 @@
+"""Planner tests (CPU): radix schedules, launch shapes and workspace sizes for the
+benchmark configurations, read back through asdFftExtDescribePlan."""
+import re
+
+import pytest
+
+from sip_amd.fft import Plan
+
+
+def steps(desc):
+    return [l.strip() for l in desc.splitlines() if l.strip().startswith("[")]
+
+
+def field(line, name):
+    m = re.search(rf"\b{name}=([\w\.x]+)", line)
+    return m.group(1) if m else None
+
+
+@pytest.mark.parametrize("n,radices", [(4096, "16x16x16"), (1000, "10x10x10"), (2187, "9x9x9x3"),
+                                       (256, "16x16"), (16, "16"), (17, "17"), (6, "6"), (2048, "16x16x8")])
+def test_radix_schedule(n, radices):
+    p = Plan("c2c", (n,), 4)
+    s = steps(p.describe())
+    assert len(s) == 1 and field(s[0], "radices") == radices
+    p.close()
+
+
+def test_one_launch_per_single_kernel_length():
+    for n in (1000, 2187, 6000, 4096, 16384):
+        p = Plan("c2c", (n,), 256)
+        assert len(steps(p.describe())) == 1 and p.workspace_size == 0
+        p.close()
+
+
+def test_large_length_is_two_pass_four_step():
+    p = Plan("c2c", (65536,), 64)
+    s = steps(p.describe())
+    assert len(s) == 2
+    assert "[in->work]" in s[0] and "[work->out]" in s[1]
+    assert p.workspace_size == 65536 * 64 * 8
+    assert p.step_bytes() == [65536 * 64 * 16.0] * 2
+    p.close()
+
+
+def test_real_even_uses_half_length_complex():
+    for kind in ("r2c", "c2r"):
+        p = Plan(kind, (8192,), 1024)
+        s = steps(p.describe())
+        assert len(s) == 1 and field(s[0], "n") == "4096"
+        p.close()
+
+
+def test_real_odd_uses_full_length():
+    p = Plan("r2c", (2187,), 2)
+    s = steps(p.describe())
+    assert field(s[0], "n") == "2187" and field(s[0], "variant") == "4"
+    p.close()
+
+
+def test_multidim_plans_have_no_transposes():
+    p = Plan("c2c", (2048, 2048), 4)
+    s = steps(p.describe())
+    assert len(s) == 2 and "[in->out]" in s[0] and "[out->out]" in s[1]
+    assert field(s[1], "variant") == "1"  # strided column pass
+    p.close()
+    p = Plan("c2c", (256, 256, 256), 1)
+    assert len(steps(p.describe())) == 3 and p.workspace_size == 0
+    p.close()
+    p = Plan("c2r", (64, 96, 80), 2)
+    s = steps(p.describe())
+    assert len(s) == 3 and "[in->work]" in s[0] and "[work->out]" in s[-1]
+    assert p.workspace_size == 2 * 64 * 96 * 41 * 8
+    p.close()
+
+
+def test_lds_within_gfx950_limit():
+    for kind, dims, b in [("c2c", (16384,), 8), ("c2c", (2048, 2048), 4), ("c2c", (4096, 4096), 1),
+                          ("r2c", (32768,), 2), ("c2c", (1 << 20,), 1), ("c2c", (19 * 17 * 13,), 3)]:
+        p = Plan(kind, dims, b)
+        for line in steps(p.describe()):
+            lds = int(field(line, "lds") or 0)
+            assert lds <= 160 * 1024, line
+            if "stockham" in line:
+                assert int(field(line, "block")) <= 512, line
+        p.close()
