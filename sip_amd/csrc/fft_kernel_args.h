@@ -62,7 +62,7 @@ struct FftArgs {
     int lds_plane;          // LDS words per plane (re plane, then im plane)
     int conj_in, conj_out;
     int big_shift;          // four-step twiddle split
-    int pad_;
+    int xcd_remap;          // 1: give each XCD a contiguous range of workgroups (shared L2 lines)
     int64_t count;          // transforms in this launch
     int64_t cols;
     int64_t in_inner, in_outer, in_stride;

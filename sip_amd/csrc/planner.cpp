@@ -1,6 +1,7 @@
 #include "planner.h"
 
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <map>
@@ -181,6 +182,16 @@ void fill_passes(const Config& c, FftArgs* a, std::vector<int>* offsets)
     }
 }
 
+// Column passes with runs shorter than a 128-byte line share lines between neighbouring
+// workgroups; keeping neighbours on one XCD lets its L2 serve the second half.
+// ASDFFT_XCD_REMAP=0/1 overrides (for measurement).
+int xcd_remap_for(bool cols_mapping, int tpb)
+{
+    const char* e = getenv("ASDFFT_XCD_REMAP");
+    if (e != nullptr && (e[0] == '0' || e[0] == '1')) return e[0] - '0';
+    return (cols_mapping && tpb < 16) ? 1 : 0;
+}
+
 // A Stockham launch.  Addressing: transform t at (t % cols)*inner + (t / cols)*outer.
 struct StepIo {
     int64_t count = 0;
@@ -222,6 +233,7 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     a.out_inner = io.out_inner;
     a.out_outer = io.out_outer;
     a.out_stride = io.out_stride;
+    a.xcd_remap = xcd_remap_for(cols_mapping, c.tpb);
     fill_passes(c, &a, &offsets);
     st->direct = false;
     st->shape.variant = variant;

@@ -427,6 +427,19 @@ constexpr int max_block()
     return EMAX <= 16 ? 256 : 512;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8),
+// and each XCD has its own L2.  With remap on, XCD x takes the contiguous logical range
+// [x*q + min(x, r), ...) of the g workgroups (q = g / 8, r = g % 8), so neighbouring tiles
+// that share 128-byte lines (64-byte column runs) are fetched into the same L2.
+__device__ __forceinline__ int64_t logical_block(int remap)
+{
+    const uint32_t b = blockIdx.x;
+    const uint32_t g = gridDim.x;
+    if (!remap || g < 16) return b;
+    const uint32_t x = b & 7u, q = g >> 3, r = g & 7u;
+    return (int64_t)x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // ------------------------------------------------------------------ kernel
 template <int EMAX, int MAP, int PRO, int EPI>
 __global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftArgs a)
@@ -443,7 +456,8 @@ __global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftAr
         tt = lane % a.tpb;
         tid = lane / a.tpb;
     }
-    const int64_t t = (int64_t)blockIdx.x * a.tpb + tt;
+    const int64_t blk = logical_block(a.xcd_remap);
+    const int64_t t = blk * a.tpb + tt;
     const bool live = t < a.count;
     Io io;
     io.in = a.in;
@@ -526,7 +540,7 @@ __global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftAr
         // strided destination (transform t -> column) is written in contiguous runs.
         const int tt2 = lane % a.tpb;
         const int step = (int)(blockDim.x / a.tpb);
-        const int64_t t2 = (int64_t)blockIdx.x * a.tpb + tt2;
+        const int64_t t2 = blk * a.tpb + tt2;
         if (t2 < a.count) {
             const int64_t c2 = t2 % a.cols;
             float2* ob = reinterpret_cast<float2*>(a.out) + c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
