@@ -43,6 +43,60 @@ def instances():
     return [(int(v), int(e)) for v, e in re.findall(r"X\((\d+),\s*(\d+)\)", body)]
 
 
+def static_instances():
+    """Rows of static_instances.txt: (variant, emax, n, threads, tpb, radices)."""
+    out = []
+    for line in open(os.path.join(CSRC, "static_instances.txt")):
+        line = line.split("#", 1)[0].strip()
+        if not line:
+            continue
+        v = [int(x) for x in line.split()]
+        out.append((v[0], v[1], v[2], v[3], v[4], tuple(v[5:])))
+    return out
+
+
+def generate_static_sources():
+    """One .hip per specialised kernel plus the host registry table (into _build/gen)."""
+    gen = os.path.join(BUILD, "gen")
+    os.makedirs(gen, exist_ok=True)
+    entries = static_instances()
+    hips = []
+    for k, (v, e, n, t, tpb, rad) in enumerate(entries):
+        src = os.path.join(gen, f"static_{k}.hip")
+        body = (
+            "// generated from static_instances.txt\n"
+            '#include "stockham_impl.h"\n'
+            "namespace asdfft {\n"
+            f"hipError_t launch_static_entry_{k}(const LaunchShape& s, const FftArgs& a, hipStream_t st)\n"
+            "{\n"
+            f"    return launch_static<{v}, {e}, {n}, {t}, {max(tpb, 1)}, {', '.join(map(str, rad))}>(s, a, st);\n"
+            "}\n"
+            "}  // namespace asdfft\n"
+        )
+        if not os.path.exists(src) or open(src).read() != body:
+            with open(src, "w") as f:
+                f.write(body)
+        hips.append(src)
+    reg = ["// generated from static_instances.txt", '#include "kernels.h"', "namespace asdfft {"]
+    for k in range(len(entries)):
+        reg.append(f"hipError_t launch_static_entry_{k}(const LaunchShape&, const FftArgs&, hipStream_t);")
+    reg.append("const StaticEntry kStaticInstances[] = {")
+    for k, (v, e, n, t, tpb, rad) in enumerate(entries):
+        reg.append(f"    {{{v}, {e}, {n}, {t}, {tpb}, {len(rad)}, {{{', '.join(map(str, rad))}}}, "
+                   f"&launch_static_entry_{k}}},")
+    if not entries:
+        reg.append("    {0, 0, 0, 0, 0, 0, {0}, nullptr},")
+    reg.append("};")
+    reg.append(f"const int kNumStaticInstances = {len(entries)};")
+    reg.append("}  // namespace asdfft")
+    reg_src = os.path.join(gen, "static_registry.cpp")
+    text = "\n".join(reg) + "\n"
+    if not os.path.exists(reg_src) or open(reg_src).read() != text:
+        with open(reg_src, "w") as f:
+            f.write(text)
+    return hips, reg_src
+
+
 def _newer(target, deps):
     if not os.path.exists(target):
         return True
@@ -73,6 +127,16 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
         if force or _newer(obj, [src] + hdr):
             tasks.append(["hipcc"] + HIPCC_FLAGS + [f"-DASDFFT_VARIANT={v}", f"-DASDFFT_EMAX={e}", "-c", src,
                                                     "-o", obj])
+    static_hips, reg_src = generate_static_sources()
+    for src in static_hips:
+        obj = src[:-4] + ".o"
+        objs.append(obj)
+        if force or _newer(obj, [src] + hdr):
+            tasks.append(["hipcc"] + HIPCC_FLAGS + ["-c", src, "-o", obj])
+    reg_obj = reg_src[:-4] + ".o"
+    objs.append(reg_obj)
+    if force or _newer(reg_obj, [reg_src] + hdr):
+        tasks.append(["g++"] + HOST_FLAGS + ["-c", reg_src, "-o", reg_obj])
     direct_obj = os.path.join(BUILD, "dft_direct.o")
     objs.append(direct_obj)
     if force or _newer(direct_obj, [os.path.join(CSRC, "dft_direct.hip")] + hdr):

@@ -91,11 +91,28 @@ ASDFFT_FOR_EACH_INSTANCE(ASDFFT_DECL)
 hipError_t launch_stockham(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     if (s.grid <= 0 || s.block <= 0 || s.block > 1024) return hipErrorInvalidConfiguration;
+    if (s.static_id >= 0) {
+        if (s.static_id >= kNumStaticInstances) return hipErrorInvalidValue;
+        return kStaticInstances[s.static_id].launch(s, args, stream);
+    }
 #define ASDFFT_CASE(V, E) \
     if (s.variant == V && s.emax == E) return launch_stockham_v##V##_e##E(s, args, stream);
     ASDFFT_FOR_EACH_INSTANCE(ASDFFT_CASE)
 #undef ASDFFT_CASE
     return hipErrorInvalidValue;
+}
+
+int find_static(int variant, int emax, int n, int threads, int tpb, const int* radices, int nradices)
+{
+    for (int i = 0; i < kNumStaticInstances; i++) {
+        const StaticEntry& e = kStaticInstances[i];
+        if (e.variant != variant || e.emax != emax || e.n != n || e.threads != threads || e.nradices != nradices) continue;
+        if (e.tpb != 0 && e.tpb != tpb) continue;
+        bool same = true;
+        for (int p = 0; p < nradices; p++) same = same && (e.radices[p] == radices[p]);
+        if (same) return i;
+    }
+    return -1;
 }
 
 }  // namespace asdfft

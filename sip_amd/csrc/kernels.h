@@ -22,6 +22,7 @@ enum Variant : int {
 };
 
 struct LaunchShape {
+    int static_id = -1;  // index into kStaticInstances, or -1 for the run-time kernel
     int variant;
     int emax;          // registers per thread (complex points): 16 or 32
     int block;         // threads per workgroup
@@ -30,6 +31,19 @@ struct LaunchShape {
 };
 
 hipError_t launch_stockham(const LaunchShape& s, const FftArgs& args, hipStream_t stream);
+
+// Specialised kernels (generated from static_instances.txt by the build).
+struct StaticEntry {
+    int variant, emax, n, threads, tpb, nradices;
+    int radices[kMaxPasses];
+    hipError_t (*launch)(const LaunchShape&, const FftArgs&, hipStream_t);
+};
+extern const StaticEntry kStaticInstances[];
+extern const int kNumStaticInstances;
+
+// Index of the specialised kernel for this launch, or -1.  tpb only has to match
+// for the column mapping (rows keep transforms-per-workgroup at run time).
+int find_static(int variant, int emax, int n, int threads, int tpb, const int* radices, int nradices);
 hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStream_t stream);
 
 }  // namespace asdfft

@@ -150,7 +150,7 @@ bool cols_config(int64_t n, int64_t count, Config* c)
         for (int emax : {16, 32}) {
             const int T = threads_for(n, radices, emax);
             if (T == 0 || tpb * T > max_block_for(emax)) continue;
-            const int64_t plane = pad16(n * tpb) + 1;
+            const int64_t plane = n * (tpb + 1);  // column layout: row pitch tpb + 1 words
             const bool need_lds = radices.size() > 1;
             const size_t lds = need_lds ? (size_t)(8 * plane) : 0;
             if (lds > kLdsLimit) continue;
@@ -241,11 +241,18 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     st->shape.block = c.tpb * c.threads;
     st->shape.lds_bytes = c.lds;
     st->shape.grid = (io.count + c.tpb - 1) / c.tpb;
+    st->shape.static_id = -1;
+    const char* ns = getenv("ASDFFT_NO_STATIC");
+    if (ns == nullptr || ns[0] != '1') {
+        st->shape.static_id = find_static(variant, c.emax, (int)n, c.threads, c.tpb, c.radices.data(),
+                                          (int)c.radices.size());
+    }
     std::ostringstream os;
     os << "stockham variant=" << variant << " n=" << n << " count=" << io.count << " radices=";
     for (size_t i = 0; i < c.radices.size(); i++) os << (i ? "x" : "") << c.radices[i];
     os << " emax=" << c.emax << " T=" << c.threads << " tpb=" << c.tpb << " block=" << st->shape.block
        << " lds=" << c.lds << " grid=" << st->shape.grid;
+    if (st->shape.static_id >= 0) os << " static=" << st->shape.static_id;
     st->describe = os.str();
     return true;
 }

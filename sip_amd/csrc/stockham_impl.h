@@ -210,16 +210,58 @@ template <> struct Dft<10> : DftCT<2, 5> {};
 template <> struct Dft<12> : DftCT<4, 3> {};
 template <> struct Dft<16> : DftCT<4, 4> {};
 
-// ------------------------------------------------------------------ LDS layout
+// ------------------------------------------------------------------ launch geometry
+// Sizes a pass needs.  The runtime kernel fills them from kernel arguments; a
+// specialised kernel builds them from template constants, so after inlining every
+// address below folds into immediate offsets (ds_read/ds_write offset fields,
+// global_load offsets) instead of per-element VALU arithmetic.
+struct Geo {
+    int n;        // complex transform length
+    int T;        // threads per transform
+    int tpb;      // transforms per workgroup
+    int tstride;  // rows mapping: padded LDS words per transform
+};
+
 __device__ __forceinline__ int pad16(int x) { return x + (x >> 4); }
 
+// LDS slot of element i of transform tt.  Rows: one padded block per transform
+// (one pad word per 16).  Columns: transforms interleaved, row pitch tpb + 1 words, so
+// the slot is linear in i and lanes that take adjacent transforms hit adjacent banks.
 template <int MAP>
-__device__ __forceinline__ int lds_at(const FftArgs& a, int tt, int i)
+__device__ __forceinline__ int lds_at(const Geo& g, int tt, int i)
 {
     if constexpr (MAP == kMapRows) {
-        return tt * a.lds_tstride + pad16(i);
+        return tt * g.tstride + pad16(i);
     } else {
-        return pad16(i * a.tpb + tt);
+        return i * (g.tpb + 1) + tt;
+    }
+}
+
+// Slot of element base + off inside the thread's own transform block (block start
+// excluded).  pbase = slot of `base`.  When off is a multiple of 16 (rows) or always
+// (columns) the slot is pbase plus a constant, which a specialised kernel folds into
+// the ds_read / ds_write immediate offset.  base16 asserts base % 16 == 0 and off < 16.
+template <int MAP, bool RT>
+__device__ __forceinline__ int lds_rel(const Geo& g, int base, int pbase, int off, bool base16)
+{
+    if constexpr (MAP == kMapRows) {
+        if constexpr (!RT) {
+            if (off % 16 == 0) return pbase + off + off / 16;
+            if (base16 && off < 16) return pbase + off;
+        }
+        return pad16(base + off);
+    } else {
+        return pbase + off * (g.tpb + 1);
+    }
+}
+
+template <int MAP>
+__device__ __forceinline__ int lds_pbase(const Geo& g, int base)
+{
+    if constexpr (MAP == kMapRows) {
+        return pad16(base);
+    } else {
+        return base * (g.tpb + 1);
     }
 }
 
@@ -228,59 +270,56 @@ __device__ __forceinline__ float2 ldg2(const float* p, int64_t idx)
     return reinterpret_cast<const float2*>(p)[idx];
 }
 
+// x.y ^= sign bit when mask is 0x80000000 (conjugation without a branch)
+__device__ __forceinline__ float2 conj_if(float2 x, uint32_t mask)
+{
+    x.y = __uint_as_float(__float_as_uint(x.y) ^ mask);
+    return x;
+}
+
 // ------------------------------------------------------------------ prologue / epilogue
 // Per-thread view of one transform: base pointers plus 32-bit element strides.
 struct Io {
     const void* in;   // first element of this transform's input
     void* out;        // first element of this transform's output
     int64_t col;      // column index (four-step twiddle)
+    uint32_t cin;     // sign mask applied to inputs (conj_in)
+    uint32_t cout;    // sign mask applied to outputs (conj_out)
+    int in_stride;    // element stride (1 for rows)
+    int out_stride;
 };
 
 template <int MAP>
-__device__ __forceinline__ int in_stride(const FftArgs& a)
+__device__ __forceinline__ int stride_of(int s)
 {
     if constexpr (MAP == kMapRows) return 1;
-    return (int)a.in_stride;
-}
-
-template <int MAP>
-__device__ __forceinline__ int out_stride(const FftArgs& a)
-{
-    if constexpr (MAP == kMapRows) return 1;
-    return (int)a.out_stride;
+    return s;
 }
 
 template <int MAP, int PRO>
-__device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int i)
+__device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int n, int i)
 {
     const float2* in2 = reinterpret_cast<const float2*>(io.in);
-    const int s = in_stride<MAP>(a);
+    const int s = stride_of<MAP>(io.in_stride);
     if constexpr (PRO == kProComplex) {
-        float2 x = in2[i * s];
-        if (a.conj_in) x.y = -x.y;
-        return x;
+        return in2[i * s];  // conjugation (conj_in) is applied by the caller
     } else if constexpr (PRO == kProReal) {
         return make_float2(reinterpret_cast<const float*>(io.in)[i * s], 0.0f);
     } else if constexpr (PRO == kProHerm) {
-        const int h = a.n >> 1;
+        const int h = n >> 1;
         float2 x;
         if (i <= h) {
             x = in2[i * s];
-            if (i == 0 || 2 * i == a.n) x.y = 0.0f;
+            if (i == 0 || 2 * i == n) x.y = 0.0f;
         } else {
-            x = in2[(a.n - i) * s];
+            x = in2[(n - i) * s];
             x.y = -x.y;
         }
-        if (a.conj_in) x.y = -x.y;
-        return x;
-    } else {  // kProC2REven: complex length M = a.n, input holds M + 1 bins
-        const int m = a.n;
-        float2 xa = in2[i * s];
-        float2 xb = in2[(m - i) * s];
-        if (a.conj_in) {
-            xa.y = -xa.y;
-            xb.y = -xb.y;
-        }
+        return conj_if(x, io.cin);
+    } else {  // kProC2REven: complex length M = n, input holds M + 1 bins
+        const int m = n;
+        float2 xa = conj_if(in2[i * s], io.cin);
+        float2 xb = conj_if(in2[(m - i) * s], io.cin);
         if (i == 0) {
             xa.y = 0.0f;
             xb.y = 0.0f;
@@ -295,50 +334,55 @@ __device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int
 }
 
 template <int MAP, int EPI>
-__device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int i, float2 x)
+__device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int n, int i, float2 x)
 {
     float2* out2 = reinterpret_cast<float2*>(io.out);
-    const int s = out_stride<MAP>(a);
+    const int s = stride_of<MAP>(io.out_stride);
     if constexpr (EPI == kEpiComplex) {
-        if (a.conj_out) x.y = -x.y;
-        out2[i * s] = x;
+        out2[i * s] = x;  // conjugation (conj_out) is applied by the caller
     } else if constexpr (EPI == kEpiHalf) {
-        if (2 * i <= a.n) {
-            if (a.conj_out) x.y = -x.y;
-            out2[i * s] = x;
-        }
+        if (2 * i <= n) out2[i * s] = x;
     } else if constexpr (EPI == kEpiReal) {
         reinterpret_cast<float*>(io.out)[i * s] = x.x;
     } else if constexpr (EPI == kEpiTwiddle) {
         const uint32_t m = (uint32_t)io.col * (uint32_t)i;  // col * i < big length <= 2^31
         const uint32_t mask = (1u << a.big_shift) - 1u;
         const float2 w = cmul(ldg2(a.tw_big_hi, m >> a.big_shift), ldg2(a.tw_big_lo, m & mask));
-        x = cmul(x, w);
-        if (a.conj_out) x.y = -x.y;
-        out2[i * s] = x;
+        out2[i * s] = conj_if(cmul(x, w), io.cout);
     }
 }
 
 // ------------------------------------------------------------------ one pass
-template <int EMAX, int R, int MAP, int PRO, int EPI>
-__device__ __forceinline__ void stockham_pass(const FftArgs& a, int p, float* lre, float* lim, int tt, int tid, bool live,
-                                              const Io& io)
+// RT: runtime geometry (the pass loop dispatches on the radix at run time).  The
+// opaque copies stop the compiler from hoisting every radix case's address
+// arithmetic out of that loop and keeping all of it live at once (hundreds of VGPRs).
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT>
+__device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
+                                              bool first, bool last, float* lre, float* lim, int tt, int tid,
+                                              bool live, const Io& io)
 {
     float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
     constexpr int B = EMAX / R;
     static_assert(B >= 1, "radix larger than the per-thread register budget");
     constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose);
-    // The pass loop calls one of many radix cases; without these opaque copies the
-    // compiler hoists every case's address arithmetic out of the loop and keeps all of
-    // it live at once (hundreds of VGPRs).  Recomputing it per pass is a few VALU ops.
-    int n = a.n;
-    int T = a.threads;
-    asm volatile("" : "+s"(n), "+s"(T));
-    asm volatile("" : "+v"(tid));
+    if constexpr (RT) {
+        asm volatile("" : "+s"(g.n), "+s"(g.T));
+        asm volatile("" : "+v"(tid));
+    }
+    const int n = g.n;
     const int nr = n / R;
-    const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
-    const bool first = (p == 0);
-    const bool last = (p == a.passes - 1);
+    const int T = g.T;
+    // this transform's LDS block (rows) or column (columns)
+    const int toff = (MAP == kMapRows) ? tt * g.tstride : tt;
+    float* lre_t = lre + toff;
+    float* lim_t = lim + toff;
+    auto kmod = [&](int j) __attribute__((always_inline)) {
+        if constexpr (RT) {
+            return j - (int)__umulhi((uint32_t)j, magic) * span;
+        } else {
+            return j % span;
+        }
+    };
 
     // 1. gather
     if (first) {
@@ -347,7 +391,13 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, int p, float* lr
             const int j = tid + b * T;
             if (live && j < nr) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, j + r * nr);
+                for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
+            }
+        }
+        if constexpr (PRO == kProComplex) {
+            if (io.cin) {  // uniform: one scalar branch instead of a XOR per point
+#pragma unroll
+                for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
             }
         }
     } else {
@@ -355,24 +405,24 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, int p, float* lr
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
             if (j < nr) {
+                const int pj = lds_pbase<MAP>(g, j);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int at = lds_at<MAP>(a, tt, j + r * nr);
-                    v[b * R + r] = make_float2(lre[at], lim[at]);
+                    const int at = lds_rel<MAP, RT>(g, j, pj, r * nr, false);
+                    v[b * R + r] = make_float2(lre_t[at], lim_t[at]);
                 }
             }
         }
     }
 
     // 2. twiddle + butterfly
-    const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
-    const float2* twp = reinterpret_cast<const float2*>(a.tw) + __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
+    const float2* twp = reinterpret_cast<const float2*>(a.tw) + tw_offset;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = tid + b * T;
         if (j < nr) {
             if (span > 1) {
-                const int k = j - (int)__umulhi((uint32_t)j, magic) * span;
+                const int k = kmod(j);
                 const float2* w = twp + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
@@ -388,43 +438,49 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, int p, float* lr
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
             if (j < nr) {
-                const int k = (span > 1) ? j - (int)__umulhi((uint32_t)j, magic) * span : 0;
+                const int k = (span > 1) ? kmod(j) : 0;
                 const int base = (j - k) * R + k;
+                const int pb = lds_pbase<MAP>(g, base);
+                const bool base16 = (span == 1) && (R % 16 == 0);  // base = j * R
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int at = lds_at<MAP>(a, tt, base + r * span);
-                    lre[at] = v[b * R + r].x;
-                    lim[at] = v[b * R + r].y;
+                    const int at = lds_rel<MAP, RT>(g, base, pb, r * span, base16);
+                    lre_t[at] = v[b * R + r].x;
+                    lim_t[at] = v[b * R + r].y;
                 }
             }
         }
         __syncthreads();
     } else {
+        if constexpr (EPI == kEpiComplex || EPI == kEpiHalf) {
+            if (io.cout) {  // uniform branch
+#pragma unroll
+                for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
+            }
+        }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
             if (live && j < nr) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) store_output<MAP, EPI>(a, io, j + r * nr, v[b * R + r]);
+                for (int r = 0; r < R; ++r) store_output<MAP, EPI>(a, io, n, j + r * nr, v[b * R + r]);
             }
         }
     }
 }
 
 template <int EMAX, int R, int MAP, int PRO, int EPI>
-__device__ __forceinline__ void maybe_pass(const FftArgs& a, int p, float* lre, float* lim, int tt, int tid, bool live,
-                                           const Io& io)
+__device__ __forceinline__ void maybe_pass(const FftArgs& a, const Geo& g, int p, float* lre, float* lim, int tt,
+                                           int tid, bool live, const Io& io)
 {
     if constexpr (R <= EMAX) {
-        stockham_pass<EMAX, R, MAP, PRO, EPI>(a, p, lre, lim, tt, tid, live, io);
+        const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
+        const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
+        const int tw_offset = __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
+        stockham_pass<EMAX, R, MAP, PRO, EPI, true>(a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre,
+                                                    lim, tt, tid, live, io);
     }
     // radices above EMAX are never scheduled: the planner checks radix <= emax before launch
-}
-
-template <int EMAX>
-constexpr int max_block()
-{
-    return EMAX <= 16 ? 256 : 512;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8),
@@ -440,29 +496,29 @@ __device__ __forceinline__ int64_t logical_block(int remap)
     return (int64_t)x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// ------------------------------------------------------------------ kernel
-template <int EMAX, int MAP, int PRO, int EPI>
-__global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftArgs a)
+// ------------------------------------------------------------------ shared kernel body
+template <int MAP, int PRO, int EPI>
+__device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int& tt, int& tid, int64_t& blk, bool& live,
+                                             Io& io)
 {
-    extern __shared__ float smem[];
-    float* lre = smem;
-    float* lim = smem + a.lds_plane;
     const int lane = threadIdx.x;
-    int tt, tid;
     if constexpr (MAP == kMapRows) {
-        tt = lane / a.threads;
-        tid = lane - tt * a.threads;
+        tt = lane / g.T;
+        tid = lane - tt * g.T;
     } else {
-        tt = lane % a.tpb;
-        tid = lane / a.tpb;
+        tt = lane % g.tpb;
+        tid = lane / g.tpb;
     }
-    const int64_t blk = logical_block(a.xcd_remap);
-    const int64_t t = blk * a.tpb + tt;
-    const bool live = t < a.count;
-    Io io;
+    blk = logical_block(a.xcd_remap);
+    const int64_t t = blk * g.tpb + tt;
+    live = t < a.count;
     io.in = a.in;
     io.out = a.out;
     io.col = 0;
+    io.cin = a.conj_in ? 0x80000000u : 0u;
+    io.cout = a.conj_out ? 0x80000000u : 0u;
+    io.in_stride = (int)a.in_stride;
+    io.out_stride = (int)a.out_stride;
     if (live) {
         io.col = t % a.cols;
         const int64_t o = t / a.cols;
@@ -479,11 +535,82 @@ __global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftAr
             io.out = reinterpret_cast<float2*>(a.out) + ob;
         }
     }
+}
+
+template <int MAP, int EPI>
+__device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
+                                             int64_t blk, bool live, const Io& io)
+{
+    if constexpr (EPI == kEpiR2CEven) {
+        // LDS holds Z = FFT_M(x[2n] + i x[2n+1]) in natural order.
+        // X[k] = 1/2 [(Z_k + conj Z_{M-k}) - i W_N^k (Z_k - conj Z_{M-k})], k in [0, M]
+        if (live) {
+            const int m = g.n;
+            float2* out2 = reinterpret_cast<float2*>(io.out);
+            for (int k = tid; 2 * k <= m; k += g.T) {
+                const int km = (k == 0) ? 0 : m - k;
+                const int at0 = lds_at<MAP>(g, tt, k);
+                const int at1 = lds_at<MAP>(g, tt, km);
+                const float2 zk = make_float2(lre[at0], lim[at0]);
+                const float2 zm = make_float2(lre[at1], lim[at1]);
+                const float2 wk = ldg2(a.tw_aux, k);
+                const float2 wm = ldg2(a.tw_aux, m - k);
+                const float2 s0 = make_float2(zk.x + zm.x, zk.y - zm.y);
+                const float2 d0 = make_float2(zk.x - zm.x, zk.y + zm.y);
+                const float2 q0 = cmul(wk, d0);
+                const float2 x0 = make_float2(0.5f * (s0.x + q0.y), 0.5f * (s0.y - q0.x));
+                const float2 s1 = make_float2(zm.x + zk.x, zm.y - zk.y);
+                const float2 d1 = make_float2(zm.x - zk.x, zm.y + zk.y);
+                const float2 q1 = cmul(wm, d1);
+                const float2 x1 = make_float2(0.5f * (s1.x + q1.y), 0.5f * (s1.y - q1.x));
+                out2[k] = conj_if(x0, io.cout);
+                if (m - k != k) out2[m - k] = conj_if(x1, io.cout);
+            }
+        }
+    } else if constexpr (EPI == kEpiTranspose) {
+        // LDS holds tpb finished transforms; adjacent lanes take adjacent transforms so the
+        // strided destination (transform t -> column) is written in contiguous runs.
+        const int lane = threadIdx.x;
+        const int tt2 = lane % g.tpb;
+        const int step = g.T;
+        const int64_t t2 = blk * g.tpb + tt2;
+        if (t2 < a.count) {
+            const int64_t c2 = t2 % a.cols;
+            float2* ob = reinterpret_cast<float2*>(a.out) + c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
+            const int s = (int)a.out_stride;
+            for (int i = lane / g.tpb; i < g.n; i += step) {
+                const int at = lds_at<MAP>(g, tt2, i);
+                ob[i * s] = conj_if(make_float2(lre[at], lim[at]), io.cout);
+            }
+        }
+    }
+}
+
+template <int EMAX>
+constexpr int max_block()
+{
+    return EMAX <= 16 ? 256 : 512;
+}
+
+// ------------------------------------------------------------------ runtime kernel
+template <int EMAX, int MAP, int PRO, int EPI>
+__global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftArgs a)
+{
+    extern __shared__ float smem[];
+    float* lre = smem;
+    float* lim = smem + a.lds_plane;
+    const Geo g{a.n, a.threads, a.tpb, a.lds_tstride};
+    int tt, tid;
+    int64_t blk;
+    bool live;
+    Io io;
+    thread_setup<MAP, PRO, EPI>(a, g, tt, tid, blk, live, io);
+
     for (int p = 0; p < a.passes; ++p) {
         // pass descriptors are uniform; readfirstlane keeps the switch a scalar branch
         switch (__builtin_amdgcn_readfirstlane(a.pass[p].radix)) {
 #define ASDFFT_CASE(RR) \
-    case RR: maybe_pass<EMAX, RR, MAP, PRO, EPI>(a, p, lre, lim, tt, tid, live, io); break;
+    case RR: maybe_pass<EMAX, RR, MAP, PRO, EPI>(a, g, p, lre, lim, tt, tid, live, io); break;
             ASDFFT_CASE(16)
             ASDFFT_CASE(12)
             ASDFFT_CASE(10)
@@ -504,55 +631,42 @@ __global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftAr
                 break;
         }
     }
+    lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
+}
 
-    if constexpr (EPI == kEpiR2CEven) {
-        // LDS holds Z = FFT_M(x[2n] + i x[2n+1]) in natural order.
-        // X[k] = 1/2 [(Z_k + conj Z_{M-k}) - i W_N^k (Z_k - conj Z_{M-k})], k in [0, M]
-        if (live) {
-            const int m = a.n;
-            float2* out2 = reinterpret_cast<float2*>(io.out);
-            for (int k = tid; 2 * k <= m; k += a.threads) {
-                const int km = (k == 0) ? 0 : m - k;
-                const int at0 = lds_at<MAP>(a, tt, k);
-                const int at1 = lds_at<MAP>(a, tt, km);
-                const float2 zk = make_float2(lre[at0], lim[at0]);
-                const float2 zm = make_float2(lre[at1], lim[at1]);
-                const float2 wk = ldg2(a.tw_aux, k);
-                const float2 wm = ldg2(a.tw_aux, m - k);
-                const float2 s0 = make_float2(zk.x + zm.x, zk.y - zm.y);
-                const float2 d0 = make_float2(zk.x - zm.x, zk.y + zm.y);
-                const float2 q0 = cmul(wk, d0);
-                float2 x0 = make_float2(0.5f * (s0.x + q0.y), 0.5f * (s0.y - q0.x));
-                const float2 s1 = make_float2(zm.x + zk.x, zm.y - zk.y);
-                const float2 d1 = make_float2(zm.x - zk.x, zm.y + zk.y);
-                const float2 q1 = cmul(wm, d1);
-                float2 x1 = make_float2(0.5f * (s1.x + q1.y), 0.5f * (s1.y - q1.x));
-                if (a.conj_out) {
-                    x0.y = -x0.y;
-                    x1.y = -x1.y;
-                }
-                out2[k] = x0;
-                if (m - k != k) out2[m - k] = x1;
-            }
-        }
-    } else if constexpr (EPI == kEpiTranspose) {
-        // LDS holds tpb finished transforms; adjacent lanes take adjacent transforms so the
-        // strided destination (transform t -> column) is written in contiguous runs.
-        const int tt2 = lane % a.tpb;
-        const int step = (int)(blockDim.x / a.tpb);
-        const int64_t t2 = blk * a.tpb + tt2;
-        if (t2 < a.count) {
-            const int64_t c2 = t2 % a.cols;
-            float2* ob = reinterpret_cast<float2*>(a.out) + c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
-            const int s = (int)a.out_stride;
-            for (int i = lane / a.tpb; i < a.n; i += step) {
-                const int at = lds_at<MAP>(a, tt2, i);
-                float2 x = make_float2(lre[at], lim[at]);
-                if (a.conj_out) x.y = -x.y;
-                ob[i * s] = x;
-            }
-        }
+// ------------------------------------------------------------------ specialised kernel
+// Same passes with N, threads, transforms per workgroup and the radix sequence as
+// template constants (instances listed in static_instances.h).
+template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
+__device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
+                                              bool live, const Io& io)
+{
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt,
+                                                 tid, live, io);
+    if constexpr (sizeof...(Rest) > 0) {
+        constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
+        static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lre, lim, tt, tid,
+                                                                                         live, io);
     }
+}
+
+template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int... Rs>
+__global__ void __launch_bounds__(max_block<EMAX>()) stockham_static_kernel(const FftArgs a)
+{
+    extern __shared__ float smem[];
+    constexpr int kTstride = N + (N >> 4) + 1;
+    float* lre = smem;
+    float* lim = smem + a.lds_plane;
+    // rows mapping keeps tpb at run time (it only offsets the transform's LDS block);
+    // the column mapping interleaves transforms inside LDS, so there it is a constant
+    const Geo g{N, T, (MAP == kMapRows) ? a.tpb : TPB, kTstride};
+    int tt, tid;
+    int64_t blk;
+    bool live;
+    Io io;
+    thread_setup<MAP, PRO, EPI>(a, g, tt, tid, blk, live, io);
+    static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, lre, lim, tt, tid, live, io);
+    lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
 }
 
 // ------------------------------------------------------------------ launch helper
@@ -566,11 +680,9 @@ template <> struct VariantTraits<kVarC2ROdd> { static constexpr int map = kMapRo
 template <> struct VariantTraits<kVarColsTwiddle> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiTwiddle; };
 template <> struct VariantTraits<kVarRowsTranspose> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiTranspose; };
 
-template <int EMAX, int V>
-hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
+template <typename K>
+hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
-    using Tr = VariantTraits<V>;
-    auto kern = stockham_kernel<EMAX, Tr::map, Tr::pro, Tr::epi>;
     if (s.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds_bytes);
@@ -578,6 +690,20 @@ hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)s.grid), dim3((unsigned)s.block), s.lds_bytes, stream, args);
     return hipGetLastError();
+}
+
+template <int EMAX, int V>
+hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
+{
+    using Tr = VariantTraits<V>;
+    return launch_kernel(stockham_kernel<EMAX, Tr::map, Tr::pro, Tr::epi>, s, args, stream);
+}
+
+template <int V, int EMAX, int N, int T, int TPB, int... Rs>
+hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
+{
+    using Tr = VariantTraits<V>;
+    return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args, stream);
 }
 
 }  // namespace asdfft
