@@ -27,7 +27,10 @@ struct LaunchShape {
     int emax;          // registers per thread (complex points): 16 or 32
     int block;         // threads per workgroup
     size_t lds_bytes;  // dynamic LDS
-    int64_t grid;      // workgroups
+    int64_t grid;      // workgroups (tiles of tpb transforms)
+    // specialised kernels are persistent: each workgroup runs at least this many tiles
+    // (more when the tiles do not all fit on the device at once)
+    int tiles_per_block = 1;
 };
 
 hipError_t launch_stockham(const LaunchShape& s, const FftArgs& args, hipStream_t stream);
@@ -37,6 +40,7 @@ struct StaticEntry {
     int variant, emax, n, threads, tpb, nradices;
     int radices[kMaxPasses];
     hipError_t (*launch)(const LaunchShape&, const FftArgs&, hipStream_t);
+    int twiddle_lds_bytes;  // LDS the kernel adds after the data planes for its twiddle table (0: none)
 };
 extern const StaticEntry kStaticInstances[];
 extern const int kNumStaticInstances;

@@ -236,3 +236,57 @@ def test_input_is_not_modified(torch_dev):
         torch.cuda.synchronize()
         assert np.array_equal(xt.cpu().numpy(), x)
         p.close()
+
+
+def _static_rows():
+    import os
+    rows = []
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sip_amd", "csrc",
+                        "static_instances.txt")
+    for line in open(path):
+        line = line.split("#", 1)[0].strip()
+        if line:
+            rows.append([int(x) for x in line.split()])
+    return rows
+
+
+def test_specialised_kernels_match_runtime_kernel(torch_dev, oracle, monkeypatch):
+    """Every compile-time specialised instance agrees with the run-time kernel and the oracle."""
+    import torch
+
+    from sip_amd.fft import Plan
+    rng = np.random.default_rng(21)
+    cases = {
+        0: lambda n: [("c2c", (n,), 48, False), ("c2c", (n,), 17, True)],
+        1: lambda n: [("c2c", (n, 64), 2, False), ("c2c", (n, 40), 1, True)],
+        2: lambda n: [("r2c", (2 * n,), 24, False), ("r2c", (2 * n,), 9, True)],
+        3: lambda n: [("c2r", (2 * n,), 24, True), ("c2r", (2 * n,), 9, False)],
+        6: lambda n: [("c2c", (n * n,), 3, False)],
+        7: lambda n: [("c2c", (n * n,), 2, True)],
+    }
+    checked = 0
+    for row in _static_rows():
+        variant, n = row[0], row[2]
+        for kind, dims, batch, inv in cases.get(variant, lambda n: [])(n):
+            shape = (batch,) + (dims[:-1] + (dims[-1] // 2 + 1,) if kind == "c2r" else dims)
+            if kind == "r2c":
+                x = rng.uniform(-1, 1, shape).astype(np.float32)
+            else:
+                x = _uniform_c(rng, shape)
+            outs = []
+            for no_static in ("0", "1"):
+                monkeypatch.setenv("ASDFFT_NO_STATIC", no_static)
+                p = Plan(kind, dims, batch, inv)
+                desc = p.describe()
+                if no_static == "1":
+                    assert "static=" not in desc, desc
+                outs.append(p.execute(torch.from_numpy(x).to(torch_dev)).cpu().numpy())
+                p.close()
+            monkeypatch.delenv("ASDFFT_NO_STATIC")
+            gold = oracle_run(oracle, kind, dims, x, inv)
+            for o in outs:
+                assert per_row_rel(o, gold, len(dims)).max() <= tol_for(dims), (row, kind, dims)
+            assert per_row_rel(outs[0], outs[1].astype(np.complex128) if np.iscomplexobj(outs[1])
+                               else outs[1].astype(np.float64), len(dims)).max() <= 1e-6
+            checked += 1
+    assert checked >= 20
