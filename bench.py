@@ -45,6 +45,37 @@ WORKLOAD = [
 ]
 
 
+def smooth_lengths(lo=64, hi=16384):
+    """Sorted {2^a 3^b 5^c} in [lo, hi]."""
+    out = []
+    p2 = 1
+    while p2 <= hi:
+        p3 = p2
+        while p3 <= hi:
+            p5 = p3
+            while p5 <= hi:
+                if p5 >= lo:
+                    out.append(p5)
+                p5 *= 5
+            p3 *= 3
+        p2 *= 2
+    return sorted(out)
+
+
+def ragged_workload(seed=10, total=4000, c2c_frac=0.7):
+    """The ragged-batch workload: `total` transforms, N log-uniform over {2^a 3^b 5^c} in
+    [64, 16384] (a uniform draw of log N snapped to the nearest such length in log
+    space), 70% forward C2C complex64 / 30% forward R2C float32.  Returns [(kind, n)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lens = np.array(smooth_lengths(), dtype=np.int64)
+    logs = np.log(lens.astype(np.float64))
+    u = rng.uniform(math.log(64), math.log(16384), total)
+    idx = np.abs(logs[None, :] - u[:, None]).argmin(axis=1)
+    kinds = np.where(rng.uniform(size=total) < c2c_frac, "c2c", "r2c")
+    return [(str(k), int(n)) for k, n in zip(kinds, lens[idx])]
+
+
 def flops(dims, batch):
     n = math.prod(dims)
     return 5.0 * n * math.log2(n) * batch
@@ -124,6 +155,104 @@ def cpu_baseline(seconds_budget=20.0):
                       f"{total_flop / 1e9:.2f} GFLOP in {dt:.1f} s"}
 
 
+def bench_ragged(torch, dev, stream, reps, warmup):
+    """Ragged-batch form: the 4000-transform workload as one persistent launch per
+    register budget (stream exec and hipGraph replay) against one plain batched plan per
+    distinct (kind, N) (host loop of C-ABI execs, and the same loop captured in a torch
+    CUDA graph).  All variants transform the same bytes; inputs resident in HBM."""
+    import collections
+
+    from sip_amd import _native as N
+    from sip_amd.fft import Plan, RaggedPlan
+    L = N.lib()
+    work = ragged_workload()
+    groups = collections.OrderedDict()
+    for k, n in sorted(work, key=lambda kn: (kn[0], kn[1])):
+        groups[(k, n)] = groups.get((k, n), 0) + 1
+    grouped = [(k, n, c, False) for (k, n), c in groups.items()]
+    scattered = [(k, n, 1, False) for k, n in work]
+    res = {"transforms": len(work), "classes": len(groups)}
+    with torch.cuda.stream(stream):
+        p_stream = RaggedPlan(grouped)
+        p_graph = RaggedPlan(grouped, graph=True)
+        p_scat = RaggedPlan(scattered, graph=True)
+        g = torch.Generator(device=dev)
+        g.manual_seed(10)
+        xin = (torch.rand(p_stream.in_extent // 4, generator=g, device=dev) * 2 - 1).view(torch.uint8)
+        yout = torch.empty(p_stream.out_extent, dtype=torch.uint8, device=dev)
+        xs = (torch.rand(p_scat.in_extent // 4, generator=g, device=dev) * 2 - 1).view(torch.uint8)
+        ys = torch.empty(p_scat.out_extent, dtype=torch.uint8, device=dev)
+        # one plain plan per class over the same packed buffers (each class is contiguous)
+        plain = []
+        ins, outs = p_stream.views(xin, "in"), p_stream.views(yout, "out")
+        for (kind, n, c, inv), xi, yo in zip(grouped, ins, outs):
+            pl = Plan(kind, (n,), c, inv)
+            pl.bind_stream(stream.cuda_stream)
+            dt_in = N.ACL_FLOAT if kind == "r2c" else N.ACL_COMPLEX64
+            dt_out = N.ACL_FLOAT if kind == "c2r" else N.ACL_COMPLEX64
+            tin = N.AclTensor(xi.shape, dt_in, xi.data_ptr())
+            tout = N.AclTensor(yo.shape, dt_out, yo.data_ptr())
+            fn = {"c2c": L.asdFftExecC2C, "r2c": L.asdFftExecR2C, "c2r": L.asdFftExecC2R}[kind]
+            plain.append((pl, tin, tout, fn))
+        torch.cuda.synchronize()
+
+        def run_plain():
+            for pl, tin, tout, fn in plain:
+                st = fn(pl.handle, tin.ptr, tout.ptr)
+                if st != 0:
+                    raise RuntimeError(f"plain exec returned {st}")
+
+        variants = {
+            "ragged_stream": lambda: p_stream.execute(xin, yout, stream.cuda_stream),
+            "ragged_graph": lambda: p_graph.execute(xin, yout, stream.cuda_stream),
+            "ragged_graph_scattered": lambda: p_scat.execute(xs, ys, stream.cuda_stream),
+            "plain_per_size_loop": run_plain,
+        }
+        # the per-size loop captured once in a CUDA graph: the plain kernels without host overhead
+        graph_ok = True
+        try:
+            run_plain()
+            torch.cuda.synchronize()
+            cg = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(cg, stream=stream):
+                run_plain()
+            variants["plain_per_size_graph"] = cg.replay
+        except Exception as e:  # capture of the plain path is a comparison aid, not the product
+            graph_ok = False
+            res["plain_graph_error"] = str(e)[:200]
+        times = {}
+        for name, fn in variants.items():
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            times[name] = event_time_ms(torch, stream, fn, reps)
+        # ragged output equals the per-size plans' output
+        run_plain()
+        torch.cuda.synchronize()
+        y_plain = yout.clone()
+        p_graph.execute(xin, yout, stream.cuda_stream)
+        torch.cuda.synchronize()
+        a = yout.view(torch.float32).double()
+        b = y_plain.view(torch.float32).double()
+        res["ragged_vs_plain_rel_l2"] = float((a - b).norm() / b.norm())
+    res["alg_bytes"] = p_stream.algorithmic_bytes
+    res["gflop"] = round(p_stream.flops / 1e9, 4)
+    for name, ms in times.items():
+        res[name] = {"us": round(ms * 1e3, 2), "gflops": round(p_stream.flops / ms / 1e6, 1),
+                     "hbm_frac": round(p_stream.algorithmic_bytes / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    best_plain = min(times["plain_per_size_loop"], times.get("plain_per_size_graph", float("inf")))
+    res["speedup_vs_best_plain"] = round(best_plain / times["ragged_graph"], 2)
+    res["plain_graph_captured"] = graph_ok
+    res["launches"] = p_stream.describe().count("launch emax")
+    for pl, tin, tout, fn in plain:
+        tin.close()
+        tout.close()
+        pl.close()
+    for p in (p_stream, p_graph, p_scat):
+        p.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--only", default=None, help="run one workload entry by name (profiling)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch form")
+    ap.add_argument("--ragged-only", action="store_true", help="run only the ragged-batch form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
@@ -147,6 +278,9 @@ def main():
     from sip_amd.fft import Plan
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
+    if a.ragged_only:
+        print(json.dumps({"forms": {"ragged": bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)}}))
+        return
     L = N.lib()
     entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
                (a.only == "c2r_n8192_b1024" and w[0] == "r2c_n8192_b1024")]
@@ -250,6 +384,8 @@ def main():
         torch.cuda.synchronize()
         line["calibration_copy_bytes"] = {"read": src.numel() * 4, "write": dst.numel() * 4}
         del src, dst
+    if not a.no_ragged:
+        line["forms"] = {"ragged": bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     for j in jobs:

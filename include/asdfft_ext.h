@@ -35,6 +35,46 @@ int32_t asdFftExtPlanSteps(void* handle, int32_t* steps, double* algorithmic_byt
 typedef struct aclTensor aclTensor;
 int32_t asdFftExtExecStep(void* handle, const aclTensor* input, const aclTensor* output, int32_t step);
 
+/* ---- ragged batches -------------------------------------------------------------------
+ * One exec transforms entries of different lengths, types and directions.  Entry e holds
+ * `count` transforms stored back to back from in_base + in_offset (inputs) and
+ * out_base + out_offset (outputs), each laid out as a plain 1D transform of its type
+ * (C2C: n complex64 in and out; R2C: n float32 in, n/2+1 complex64 out; C2R: the
+ * reverse).  No normalisation, the same sign convention as asdFftExecC2C / R2C / C2R.
+ * The reference has no counterpart: there, each distinct (type, length) needs its own
+ * plan and exec (core/fft/fft_api.cpp:1124-1190, 1752-1844). */
+typedef struct asdFftExtRaggedEntry {
+    int32_t type;        /* asdFftType value: C2C 0x10, C2R 0x11, R2C 0x12 */
+    int32_t direction;   /* asdFftDirection value: FORWARD 0x10, INVERSE 0x11 */
+    int64_t n;           /* transform length (the real length for R2C / C2R) */
+    int64_t count;       /* transforms in this entry */
+    int64_t in_offset;   /* bytes from the exec's input base to the first input */
+    int64_t out_offset;  /* bytes from the exec's output base to the first output */
+} asdFftExtRaggedEntry;
+
+#define ASDFFT_RAGGED_GRAPH 0x1u    /* exec replays an instantiated hipGraph of the launches */
+#define ASDFFT_RAGGED_AUTOTUNE 0x2u /* time candidate schedules on the device while planning */
+
+/* No reference counterpart (batching form).  Plans a ragged batch on a handle from
+ * asdFftCreate.  Outputs of different entries must not overlap.  Returns
+ * ACL_ERROR_INVALID_PARAM for invalid entries and ACL_ERROR_API_NOT_SUPPORT for lengths
+ * outside the ragged kernels (complex pass length above 16384 or a prime factor above
+ * 19).  With ASDFFT_RAGGED_AUTOTUNE the plan call runs short timing launches on the
+ * current device (and allocates scratch for them). */
+int32_t asdFftExtMakePlanRagged(void* handle, const asdFftExtRaggedEntry* entries, int64_t num_entries,
+                                uint32_t flags);
+
+/* No reference counterpart (batching form).  Enqueues the ragged batch on the handle's
+ * stream (asdFftSetStream).  Inputs and outputs must not overlap, except that an
+ * all-C2C batch whose entries have in_offset == out_offset may run in place with
+ * in_base == out_base.  Execs of one plan share its tile queue: run them on one stream. */
+int32_t asdFftExtExecRagged(void* handle, const void* in_base, void* out_base);
+
+/* No reference counterpart (batching form).  Bytes past each base that an exec reads
+ * and writes, the algorithmic bytes (inputs read once + outputs written once) and the
+ * flop count 5 n log2 n summed over the transforms. */
+int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t* out_extent, double* bytes, double* flops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,21 @@ class FftError(RuntimeError):
 
 _lib = None
 
+ASDFFT_RAGGED_GRAPH = 0x1
+ASDFFT_RAGGED_AUTOTUNE = 0x2
+
+
+class RaggedEntry(ctypes.Structure):
+    """asdFftExtRaggedEntry (include/asdfft_ext.h)."""
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("direction", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+        ("count", ctypes.c_int64),
+        ("in_offset", ctypes.c_int64),
+        ("out_offset", ctypes.c_int64),
+    ]
+
 # (name, restype, argtypes) of every exported entry point
 _VP = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -97,6 +112,10 @@ SIGNATURES = [
     ("asdFftExtDescribePlan", _I32, [_VP, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("asdFftExtPlanSteps", _I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     ("asdFftExtExecStep", _I32, [_VP, _VP, _VP, _I32]),
+    ("asdFftExtMakePlanRagged", _I32, [_VP, ctypes.POINTER(RaggedEntry), _I64, ctypes.c_uint32]),
+    ("asdFftExtExecRagged", _I32, [_VP, _VP, _VP]),
+    ("asdFftExtRaggedInfo", _I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double)]),
 ]
 
 

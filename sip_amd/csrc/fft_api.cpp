@@ -22,6 +22,7 @@
 #include "acl_meta.h"
 #include "asdfft_ext.h"
 #include "planner.h"
+#include "ragged.h"
 #include "tensor_desc.h"
 
 namespace {
@@ -46,6 +47,7 @@ struct FftPlan {
     size_t workspace_recorded = 0;  // set by GetWorkspaceSize (fft_api.cpp:1339-1360)
     bool workspace_queried = false;
     asdfft::BuiltPlan built;
+    std::unique_ptr<asdfft::RaggedPlan> ragged;  // set by asdFftExtMakePlanRagged
 };
 
 std::mutex g_table_mu;
@@ -271,6 +273,10 @@ AspbStatus exec_common(asdFftHandle handle, const aclTensor* input, const aclTen
     std::lock_guard<std::mutex> lock(p.mu);
     if (p.state != PlanState::kInitialized) {
         log_error(fn, "plan is not initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (p.ragged) {
+        log_error(fn, "ragged-batch plans run through asdFftExtExecRagged");
         return ACL_ERROR_INVALID_PARAM;
     }
     if (p.type != want) {  // fft_api.cpp:1764-1767
@@ -609,7 +615,7 @@ extern "C" int32_t asdFftExtDescribePlan(void* handle, char* buffer, size_t leng
     auto sp = lookup(handle);
     if (!sp) return AsdSip::ACL_ERROR_INVALID_PARAM;
     std::lock_guard<std::mutex> lock(sp->mu);
-    const std::string s = asdfft::describe_plan(sp->built);
+    const std::string s = sp->ragged ? sp->ragged->describe : asdfft::describe_plan(sp->built);
     if (needed != nullptr) *needed = s.size() + 1;
     if (buffer != nullptr && length > 0) {
         const size_t n = std::min(length - 1, s.size());
@@ -624,6 +630,18 @@ extern "C" int32_t asdFftExtPlanSteps(void* handle, int32_t* steps, double* algo
     auto sp = lookup(handle);
     if (!sp) return AsdSip::ACL_ERROR_INVALID_PARAM;
     std::lock_guard<std::mutex> lock(sp->mu);
+    if (sp->ragged) {
+        const asdfft::RaggedPlan& r = *sp->ragged;
+        if (steps != nullptr) *steps = (int32_t)r.launches.size();
+        if (algorithmic_bytes != nullptr) {
+            for (size_t i = 0; i < r.launches.size(); i++) {
+                double b = 0;
+                for (int ci : r.launches[i].class_ids) b += r.classes[ci].bytes;
+                algorithmic_bytes[i] = b;
+            }
+        }
+        return AsdSip::ACL_SUCCESS;
+    }
     if (steps != nullptr) *steps = (int32_t)sp->built.steps.size();
     if (algorithmic_bytes != nullptr) {
         for (size_t i = 0; i < sp->built.steps.size(); i++) algorithmic_bytes[i] = sp->built.steps[i].bytes;
@@ -646,4 +664,135 @@ extern "C" int32_t asdFftExtExecStep(void* handle, const aclTensor* input, const
     if (t == AsdSip::ASCEND_FFT_C2R)
         return exec_common(handle, input, output, t, match_half_complex, match_real, "asdFftExtExecStep", step);
     return exec_common(handle, input, output, t, match_c2c, match_c2c, "asdFftExtExecStep", step);
+}
+
+extern "C" int32_t asdFftExtMakePlanRagged(void* handle, const asdFftExtRaggedEntry* entries, int64_t num_entries,
+                                           uint32_t flags)
+{
+    const char* fn = "asdFftExtMakePlanRagged";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state == PlanState::kInitialized) {
+        log_error(fn, "plan is already initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (entries == nullptr || num_entries <= 0 || num_entries > std::numeric_limits<int32_t>::max()) {
+        log_error(fn, "entries must be a non-empty array");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if ((flags & ~(ASDFFT_RAGGED_GRAPH | ASDFFT_RAGGED_AUTOTUNE)) != 0) {
+        log_error(fn, "unknown flags");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    std::vector<asdfft::RaggedEntry> list;
+    list.reserve((size_t)num_entries);
+    for (int64_t i = 0; i < num_entries; i++) {
+        const asdFftExtRaggedEntry& e = entries[i];
+        asdfft::RaggedEntry r;
+        if (e.type == ASCEND_FFT_C2C) {
+            r.kind = asdfft::RaggedKind::C2C;
+        } else if (e.type == ASCEND_FFT_R2C) {
+            r.kind = asdfft::RaggedKind::R2C;
+        } else if (e.type == ASCEND_FFT_C2R) {
+            r.kind = asdfft::RaggedKind::C2R;
+        } else {
+            log_error(fn, "entry " + std::to_string(i) + ": unsupported type");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+        if (!valid_direction(static_cast<asdFftDirection>(e.direction))) {
+            log_error(fn, "entry " + std::to_string(i) + ": invalid direction");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+        if (e.n <= 0 || e.n > kMaxFftSize || e.count <= 0 || e.count > std::numeric_limits<int32_t>::max()) {
+            log_error(fn, "entry " + std::to_string(i) + ": invalid length or count");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+        r.inverse = (e.direction == ASCEND_FFT_INVERSE);
+        r.n = e.n;
+        r.count = e.count;
+        r.in_offset = e.in_offset;
+        r.out_offset = e.out_offset;
+        list.push_back(r);
+    }
+    asdfft::RaggedOptions opt;
+    opt.graph = (flags & ASDFFT_RAGGED_GRAPH) != 0;
+    opt.autotune = (flags & ASDFFT_RAGGED_AUTOTUNE) != 0;
+    auto plan = std::make_unique<asdfft::RaggedPlan>();
+    bool unsupported = false;
+    std::string why;
+    try {
+        why = asdfft::build_ragged(list, opt, plan.get(), &unsupported);
+    } catch (const std::exception& e) {
+        why = e.what();
+    }
+    if (!why.empty()) {
+        log_error(fn, why);
+        return unsupported ? ACL_ERROR_API_NOT_SUPPORT : ACL_ERROR_INVALID_PARAM;
+    }
+    p.type = static_cast<asdFftType>(entries[0].type);
+    p.direction = static_cast<asdFftDirection>(entries[0].direction);
+    if (trace_enabled()) fprintf(stderr, "[asdfft] %s", plan->describe.c_str());
+    p.ragged = std::move(plan);
+    p.state = PlanState::kInitialized;
+    return ACL_SUCCESS;
+}
+
+extern "C" int32_t asdFftExtExecRagged(void* handle, const void* in_base, void* out_base)
+{
+    const char* fn = "asdFftExtExecRagged";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state != PlanState::kInitialized || !p.ragged) {
+        log_error(fn, "handle holds no ragged-batch plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    asdfft::RaggedPlan& r = *p.ragged;
+    if (in_base == nullptr || out_base == nullptr) {
+        log_error(fn, "data pointer is nullptr");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    const bool in_place = (in_base == out_base) && r.all_inplace_c2c;
+    if (!in_place && ranges_overlap(in_base, r.in_extent, out_base, r.out_extent)) {
+        log_error(fn, "input and output ranges overlap");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!asdfft::device_available()) {
+        log_error(fn, "no HIP device: this library has no host execution path");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    if (!r.ready) {
+        log_error(fn, "plan was built without a HIP device; create it again on a device");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    const hipError_t e = asdfft::exec_ragged(r, in_base, out_base, static_cast<hipStream_t>(p.stream));
+    if (e != hipSuccess) {
+        log_error(fn, std::string("launch failed: ") + hipGetErrorString(e));
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    return ACL_SUCCESS;
+}
+
+extern "C" int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t* out_extent, double* bytes,
+                                       double* flops)
+{
+    auto sp = lookup(handle);
+    if (!sp) return ACL_ERROR_INVALID_PARAM;
+    std::lock_guard<std::mutex> lock(sp->mu);
+    if (!sp->ragged) return ACL_ERROR_INVALID_PARAM;
+    const asdfft::RaggedPlan& r = *sp->ragged;
+    if (in_extent != nullptr) *in_extent = (int64_t)r.in_extent;
+    if (out_extent != nullptr) *out_extent = (int64_t)r.out_extent;
+    if (bytes != nullptr) *bytes = r.bytes;
+    if (flops != nullptr) *flops = r.flops;
+    return ACL_SUCCESS;
 }

@@ -71,6 +71,33 @@ struct FftArgs {
     PassDesc pass[kMaxPasses];
 };
 
+// ---- ragged batches: transforms of different lengths and kinds in one launch ----------
+// A class is every transform of one (kind, direction, length) in the batch; its FftArgs
+// carry the geometry, pass list, twiddle tables and conjugation (in/out/count unused).
+struct RaggedClass {
+    FftArgs a;
+    int variant;   // kVarRowsC2C, kVarR2CEven, kVarC2REven, kVarR2COdd or kVarC2ROdd
+    int reserved;
+};
+
+// A workgroup's unit of work: `count` (<= the class's tpb) transforms of one class whose
+// byte offsets are entries first .. first + count - 1 of the offset table.
+struct RaggedTile {
+    int cls;
+    int count;
+    int64_t first;
+};
+
+// Kernel arguments besides the class and tile tables (those are separate restrict
+// pointers so the uniform descriptor reads can use scalar loads).
+struct RaggedArgs {
+    const void* in_base;
+    void* out_base;
+    const int64_t* offsets;  // per transform: {input byte offset, output byte offset}
+    int* queue;              // {next tile, workgroups finished}: both zero between launches
+    int ntiles;
+};
+
 // Direct O(N^2) DFT kernel parameters (lengths with factors outside the radix set, N == 1).
 struct DftArgs {
     const void* in;

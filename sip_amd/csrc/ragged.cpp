@@ -1,0 +1,505 @@
+// Host planner of ragged batches (see ragged.h).
+//
+// Entries with the same (kind, direction, length) form a class.  Each class gets a
+// Stockham schedule (radix order, threads per transform, transforms per tile) and an
+// EMAX register budget; classes with one budget share one persistent launch.  Every
+// class's transforms are listed in the offset table, cut into tiles of tpb transforms,
+// and the tiles of a launch are sorted by decreasing cost (longest-processing-time
+// first) so the dynamic tile queue ends with the cheapest tiles.
+#include "ragged.h"
+
+#include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <sstream>
+
+#include "planner.h"
+
+namespace asdfft {
+
+namespace {
+
+constexpr size_t kLdsLimit = 160 * 1024;
+constexpr size_t kLdsFloor = 64 * 1024;
+constexpr int64_t kMaxComplexLength = 16384;  // one class's transform fits one workgroup's LDS
+constexpr int kQueueStrideInts = 32;          // 128 bytes between launches' queue words
+
+int64_t pad16(int64_t x) { return x + (x >> 4); }
+
+uint32_t magic_for(int span)
+{
+    if (span <= 1) return 0;
+    return (uint32_t)((((uint64_t)1) << 32) / (uint64_t)span + 1);
+}
+
+std::string table_key(int64_t n, const std::vector<int>& r)
+{
+    std::ostringstream os;
+    os << "stk:" << n;
+    for (int v : r) os << ":" << v;
+    return os.str();
+}
+
+const char* kind_name(RaggedKind k)
+{
+    switch (k) {
+        case RaggedKind::C2C: return "c2c";
+        case RaggedKind::R2C: return "r2c";
+        default: return "c2r";
+    }
+}
+
+int64_t in_bytes_of(const RaggedEntry& e)
+{
+    switch (e.kind) {
+        case RaggedKind::C2C: return 8 * e.n;
+        case RaggedKind::R2C: return 4 * e.n;
+        default: return 8 * (e.n / 2 + 1);
+    }
+}
+
+int64_t out_bytes_of(const RaggedEntry& e)
+{
+    switch (e.kind) {
+        case RaggedKind::C2C: return 8 * e.n;
+        case RaggedKind::R2C: return 8 * (e.n / 2 + 1);
+        default: return 4 * e.n;
+    }
+}
+
+// Kernel variant, complex pass length, conjugation and load alignment of a class
+// (the same choices the batched planner makes for a plain 1D plan).
+struct ClassShape {
+    int variant;
+    int64_t n;  // complex length of the passes
+    int conj_in, conj_out;
+    bool aux;   // needs the half-length W_N^k table
+    int in_align, out_align;
+};
+
+bool shape_of(const RaggedEntry& e, ClassShape* s)
+{
+    const bool inv = e.inverse;
+    if (e.kind == RaggedKind::C2C) {
+        *s = {kVarRowsC2C, e.n, inv, inv, false, 8, 8};
+        return true;
+    }
+    const bool even = (e.n % 2) == 0 && e.n >= 4 && !choose_radices(e.n / 2).empty();
+    if (e.kind == RaggedKind::R2C) {
+        if (even) {
+            *s = {kVarR2CEven, e.n / 2, 0, inv, true, 8, 8};
+        } else {
+            *s = {kVarR2COdd, e.n, 0, inv, false, 4, 8};
+        }
+    } else {
+        if (even) {
+            *s = {kVarC2REven, e.n / 2, inv ? 0 : 1, 1, true, 8, 8};
+        } else {
+            *s = {kVarC2ROdd, e.n, inv, 0, false, 8, 4};
+        }
+    }
+    return true;
+}
+
+std::string class_key(const RaggedEntry& e)
+{
+    std::ostringstream os;
+    os << kind_name(e.kind) << ":" << (e.inverse ? "i" : "f") << ":" << e.n;
+    return os.str();
+}
+
+// Threads per transform so that every pass has EMAX / R butterflies per thread at most.
+int min_threads(int64_t n, const std::vector<int>& radices, int emax)
+{
+    int64_t t = 0;
+    for (int r : radices) {
+        if (r > emax) return 0;
+        const int64_t b = emax / r;
+        t = std::max(t, (n / r + b - 1) / b);
+    }
+    return (int)t;
+}
+
+bool need_lds(const RaggedClassInfo& c)
+{
+    return c.sched.radices.size() > 1 || c.variant == kVarR2CEven;
+}
+
+size_t class_lds(const RaggedClassInfo& c, int tpb)
+{
+    return need_lds(c) ? (size_t)8 * (size_t)tpb * (size_t)(pad16(c.n) + 1) : 0;
+}
+
+int device_cus()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                                 hipSuccess && n > 0) {
+        return n;
+    }
+    return 256;
+}
+
+void free_device(void* p)
+{
+    if (p != nullptr) (void)hipFree(p);
+}
+
+}  // namespace
+
+RaggedPlan::~RaggedPlan()
+{
+    int prev = -1;
+    const bool have = device >= 0 && hipGetDevice(&prev) == hipSuccess;
+    if (have && prev != device) (void)hipSetDevice(device);
+    if (graph_exec != nullptr) (void)hipGraphExecDestroy(graph_exec);
+    if (graph != nullptr) (void)hipGraphDestroy(graph);
+    for (RaggedLaunchPlan& l : launches) {
+        free_device(l.d_classes);
+        free_device(l.d_tiles);
+    }
+    free_device(d_offsets);
+    free_device(d_queue);
+    if (have && prev != device) (void)hipSetDevice(prev);
+}
+
+std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOptions& opt, RaggedPlan* plan,
+                         bool* unsupported)
+{
+    *unsupported = false;
+    plan->entries = entries;
+    plan->opt = opt;
+    if (entries.empty()) return "a ragged batch needs at least one entry";
+
+    // ---- validation, extents, classes
+    std::map<std::string, int> class_of;
+    std::vector<std::vector<int>> class_entries;
+    std::vector<ClassShape> shapes;
+    std::vector<std::pair<int64_t, int64_t>> out_ranges;
+    bool inplace = true;
+    for (size_t i = 0; i < entries.size(); i++) {
+        const RaggedEntry& e = entries[i];
+        const std::string where = "entry " + std::to_string(i) + ": ";
+        if (e.n < 1 || e.count < 1) return where + "length and count must be positive";
+        if (e.in_offset < 0 || e.out_offset < 0) return where + "offsets must be non-negative";
+        ClassShape s;
+        shape_of(e, &s);
+        if (e.in_offset % s.in_align != 0 || e.out_offset % s.out_align != 0) {
+            return where + "offsets must be multiples of the element size (" + std::to_string(s.in_align) + " / " +
+                   std::to_string(s.out_align) + " bytes)";
+        }
+        if (s.n < 2 || choose_radices(s.n).empty() || s.n > kMaxComplexLength) {
+            *unsupported = true;
+            return where + "length " + std::to_string(e.n) +
+                   " is outside the ragged kernels (radix-{2..19}-smooth, at most 16384 complex points per pass)";
+        }
+        const int64_t ib = in_bytes_of(e), ob = out_bytes_of(e);
+        plan->in_extent = std::max<size_t>(plan->in_extent, (size_t)(e.in_offset + e.count * ib));
+        plan->out_extent = std::max<size_t>(plan->out_extent, (size_t)(e.out_offset + e.count * ob));
+        out_ranges.push_back({e.out_offset, e.out_offset + e.count * ob});
+        inplace = inplace && e.kind == RaggedKind::C2C && e.in_offset == e.out_offset;
+        const std::string key = class_key(e);
+        auto it = class_of.find(key);
+        if (it == class_of.end()) {
+            class_of[key] = (int)plan->classes.size();
+            RaggedClassInfo c;
+            c.key = key;
+            c.variant = s.variant;
+            c.n = s.n;
+            c.nreal = e.n;
+            plan->classes.push_back(c);
+            class_entries.push_back({});
+            shapes.push_back(s);
+            it = class_of.find(key);
+        }
+        RaggedClassInfo& c = plan->classes[it->second];
+        c.transforms += e.count;
+        c.bytes += (double)(ib + ob) * (double)e.count;
+        class_entries[it->second].push_back((int)i);
+        plan->bytes += (double)(ib + ob) * (double)e.count;
+        plan->flops += 5.0 * (double)e.n * log2((double)e.n) * (double)e.count;
+    }
+    std::sort(out_ranges.begin(), out_ranges.end());
+    for (size_t i = 1; i < out_ranges.size(); i++) {
+        if (out_ranges[i].first < out_ranges[i - 1].second) return "output ranges of two entries overlap";
+    }
+    plan->all_inplace_c2c = inplace;
+
+    // ---- per-class schedules
+    for (size_t ci = 0; ci < plan->classes.size(); ci++) {
+        RaggedClassInfo& c = plan->classes[ci];
+        auto f = opt.fixed.find(c.key);
+        ClassSchedule sch;
+        if (f != opt.fixed.end()) {
+            sch = f->second;
+        } else {
+            sch.radices = choose_radices(c.n);
+        }
+        int64_t prod = 1;
+        for (int r : sch.radices) prod *= r;
+        if (prod != c.n || sch.radices.empty() || (int)sch.radices.size() > kMaxPasses) {
+            return "schedule for " + c.key + " does not factor its length";
+        }
+        c.emax = 0;
+        for (int emax : {16, 32}) {
+            const int t = min_threads(c.n, sch.radices, emax);
+            if (t > 0 && t <= kRaggedMaxBlock) {
+                c.emax = emax;
+                sch.threads = std::max(sch.threads, t);
+                break;
+            }
+        }
+        if (c.emax == 0 || sch.threads > kRaggedMaxBlock) {
+            *unsupported = true;
+            return "no register budget holds " + c.key;
+        }
+        c.sched = sch;
+    }
+
+    // ---- launches: one per register budget
+    for (int emax : {16, 32}) {
+        RaggedLaunchPlan L;
+        L.emax = emax;
+        int maxT = 0;
+        size_t one = 0;
+        for (size_t ci = 0; ci < plan->classes.size(); ci++) {
+            if (plan->classes[ci].emax != emax) continue;
+            L.class_ids.push_back((int)ci);
+            maxT = std::max(maxT, plan->classes[ci].sched.threads);
+            one = std::max(one, class_lds(plan->classes[ci], 1));
+        }
+        if (L.class_ids.empty()) continue;
+        int block = (emax == 16 && opt.block16 > 0) ? opt.block16 : 512;
+        block = std::max(block, (maxT + 63) / 64 * 64);
+        block = std::min(block, kRaggedMaxBlock);
+        L.block = block;
+        const size_t budget = std::min(kLdsLimit, std::max(kLdsFloor, one));
+        for (int ci : L.class_ids) {
+            RaggedClassInfo& c = plan->classes[ci];
+            int tpb = block / c.sched.threads;
+            if (c.sched.tpb > 0) tpb = std::min(tpb, c.sched.tpb);
+            tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, c.transforms));
+            while (tpb > 1 && class_lds(c, tpb) > budget) tpb--;
+            c.sched.tpb = tpb;
+            c.lds = class_lds(c, tpb);
+            L.lds_bytes = std::max(L.lds_bytes, c.lds);
+        }
+        plan->launches.push_back(L);
+    }
+
+    // ---- offset table and tiles
+    for (RaggedLaunchPlan& L : plan->launches) {
+        struct Costed {
+            double cost;
+            RaggedTile t;
+        };
+        std::vector<Costed> tiles;
+        for (size_t k = 0; k < L.class_ids.size(); k++) {
+            const int ci = L.class_ids[k];
+            const RaggedClassInfo& c = plan->classes[ci];
+            const int64_t first = (int64_t)plan->offsets.size() / 2;
+            for (int ei : class_entries[ci]) {
+                const RaggedEntry& e = entries[ei];
+                const int64_t ib = in_bytes_of(e), ob = out_bytes_of(e);
+                for (int64_t t = 0; t < e.count; t++) {
+                    plan->offsets.push_back(e.in_offset + t * ib);
+                    plan->offsets.push_back(e.out_offset + t * ob);
+                }
+            }
+            const double per = (double)c.n * (double)(c.sched.radices.size() + 1);
+            for (int64_t f = 0; f < c.transforms; f += c.sched.tpb) {
+                RaggedTile t;
+                t.cls = (int)k;  // index into this launch's class table
+                t.count = (int)std::min<int64_t>(c.sched.tpb, c.transforms - f);
+                t.first = first + f;
+                tiles.push_back({per * t.count, t});
+            }
+        }
+        std::stable_sort(tiles.begin(), tiles.end(), [](const Costed& a, const Costed& b) { return a.cost > b.cost; });
+        for (const Costed& t : tiles) L.tiles.push_back(t.t);
+        L.ntiles = (int64_t)L.tiles.size();
+    }
+
+    // ---- description
+    std::ostringstream os;
+    os << "ragged plan: " << entries.size() << " entries, " << plan->classes.size() << " classes, "
+       << plan->launches.size() << " launch(es), " << (plan->opt.graph ? "graph" : "stream") << " exec\n";
+    for (const RaggedLaunchPlan& L : plan->launches) {
+        os << "  launch emax=" << L.emax << " block=" << L.block << " lds=" << L.lds_bytes << " tiles=" << L.ntiles
+           << "\n";
+        for (int ci : L.class_ids) {
+            const RaggedClassInfo& c = plan->classes[ci];
+            os << "    " << c.key << " variant=" << c.variant << " n=" << c.n << " radices=";
+            for (size_t i = 0; i < c.sched.radices.size(); i++) os << (i ? "x" : "") << c.sched.radices[i];
+            os << " T=" << c.sched.threads << " tpb=" << c.sched.tpb << " transforms=" << c.transforms << "\n";
+        }
+    }
+    plan->describe = os.str();
+
+    // ---- device tables (skipped without a device: the plan still validates)
+    if (!device_available()) return "";
+    if (hipGetDevice(&plan->device) != hipSuccess) return "";
+    for (RaggedLaunchPlan& L : plan->launches) {
+        for (int ci : L.class_ids) {
+            const RaggedClassInfo& c = plan->classes[ci];
+            const ClassShape& s = shapes[ci];
+            RaggedClass rc{};
+            FftArgs& a = rc.a;
+            std::vector<int> offs;
+            TablePtr tw = upload_table(table_key(c.n, c.sched.radices), stockham_table((int)c.n, c.sched.radices, &offs));
+            if (!tw) return "";
+            plan->tables.push_back(tw);
+            a.tw = tw->dev;
+            if (s.aux) {
+                TablePtr aux = upload_table("aux:" + std::to_string(c.nreal), roots_table(c.nreal, c.nreal / 2 + 1));
+                if (!aux) return "";
+                plan->tables.push_back(aux);
+                a.tw_aux = aux->dev;
+            }
+            a.n = (int)c.n;
+            a.threads = c.sched.threads;
+            a.tpb = c.sched.tpb;
+            a.lds_tstride = (int)(pad16(c.n) + 1);
+            a.lds_plane = c.sched.tpb * a.lds_tstride;
+            a.conj_in = s.conj_in;
+            a.conj_out = s.conj_out;
+            a.count = c.transforms;
+            a.cols = 1;
+            a.in_stride = 1;
+            a.out_stride = 1;
+            a.passes = (int)c.sched.radices.size();
+            int span = 1;
+            for (size_t p = 0; p < c.sched.radices.size(); p++) {
+                a.pass[p].radix = c.sched.radices[p];
+                a.pass[p].span = span;
+                a.pass[p].magic = magic_for(span);
+                a.pass[p].tw_offset = offs[p];
+                span *= c.sched.radices[p];
+            }
+            rc.variant = c.variant;
+            L.dev_classes.push_back(rc);
+        }
+        if (hipMalloc(reinterpret_cast<void**>(&L.d_classes), L.dev_classes.size() * sizeof(RaggedClass)) !=
+                hipSuccess ||
+            hipMemcpy(L.d_classes, L.dev_classes.data(), L.dev_classes.size() * sizeof(RaggedClass),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            return "device allocation failed";
+        }
+        if (hipMalloc(reinterpret_cast<void**>(&L.d_tiles), L.tiles.size() * sizeof(RaggedTile)) != hipSuccess ||
+            hipMemcpy(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(RaggedTile), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            return "device allocation failed";
+        }
+        int per_cu = 0;
+        if (ragged_prepare(L.emax, L.lds_bytes, L.block, &per_cu) != hipSuccess || per_cu < 1) {
+            return "the ragged kernel cannot run with " + std::to_string(L.lds_bytes) + " bytes of LDS";
+        }
+        L.grid = std::min<int64_t>(L.ntiles, (int64_t)per_cu * device_cus());
+    }
+    const size_t obytes = plan->offsets.size() * sizeof(int64_t);
+    if (hipMalloc(reinterpret_cast<void**>(&plan->d_offsets), obytes) != hipSuccess ||
+        hipMemcpy(plan->d_offsets, plan->offsets.data(), obytes, hipMemcpyHostToDevice) != hipSuccess) {
+        return "device allocation failed";
+    }
+    const size_t qbytes = plan->launches.size() * kQueueStrideInts * sizeof(int);
+    if (hipMalloc(reinterpret_cast<void**>(&plan->d_queue), qbytes) != hipSuccess ||
+        hipMemset(plan->d_queue, 0, qbytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        return "device allocation failed";
+    }
+    plan->ready = true;
+    return "";
+}
+
+namespace {
+
+RaggedArgs args_for(const RaggedPlan& p, size_t li, const void* in, void* out)
+{
+    RaggedArgs a{};
+    a.in_base = in;
+    a.out_base = out;
+    a.offsets = p.d_offsets;
+    a.queue = p.d_queue + li * kQueueStrideInts;
+    a.ntiles = (int)p.launches[li].ntiles;
+    return a;
+}
+
+hipError_t node_params(const RaggedPlan& p, size_t li, RaggedArgs* args, void** slots, hipKernelNodeParams* kp)
+{
+    const RaggedLaunchPlan& L = p.launches[li];
+    slots[0] = args;
+    slots[1] = const_cast<RaggedClass**>(&L.d_classes);
+    slots[2] = const_cast<RaggedTile**>(&L.d_tiles);
+    *kp = hipKernelNodeParams{};
+    kp->func = const_cast<void*>(ragged_kernel_fn(L.emax));
+    kp->gridDim = dim3((unsigned)L.grid);
+    kp->blockDim = dim3((unsigned)L.block);
+    kp->sharedMemBytes = (unsigned)L.lds_bytes;
+    kp->kernelParams = slots;
+    kp->extra = nullptr;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t exec_ragged(RaggedPlan& p, const void* in, void* out, hipStream_t stream)
+{
+    if (!p.ready) return hipErrorNotReady;
+    if (!p.opt.graph) {
+        for (size_t li = 0; li < p.launches.size(); li++) {
+            const RaggedLaunchPlan& L = p.launches[li];
+            const RaggedArgs a = args_for(p, li, in, out);
+            hipError_t e = launch_ragged(L.emax, L.grid, L.block, L.lds_bytes, a, L.d_classes, L.d_tiles, stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    hipError_t e;
+    if (p.graph_exec == nullptr) {
+        e = hipGraphCreate(&p.graph, 0);
+        if (e != hipSuccess) return e;
+        // the launches touch disjoint outputs and queues: independent nodes
+        for (size_t li = 0; li < p.launches.size(); li++) {
+            RaggedArgs a = args_for(p, li, in, out);
+            void* slots[3];
+            hipKernelNodeParams kp;
+            node_params(p, li, &a, slots, &kp);
+            hipGraphNode_t node;
+            e = hipGraphAddKernelNode(&node, p.graph, nullptr, 0, &kp);
+            if (e != hipSuccess) return e;
+            p.nodes.push_back(node);
+        }
+        e = hipGraphInstantiate(&p.graph_exec, p.graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            p.graph_exec = nullptr;
+            return e;
+        }
+        p.graph_in = in;
+        p.graph_out = out;
+    } else if (in != p.graph_in || out != p.graph_out) {
+        for (size_t li = 0; li < p.launches.size(); li++) {
+            RaggedArgs a = args_for(p, li, in, out);
+            void* slots[3];
+            hipKernelNodeParams kp;
+            node_params(p, li, &a, slots, &kp);
+            e = hipGraphExecKernelNodeSetParams(p.graph_exec, p.nodes[li], &kp);
+            if (e != hipSuccess) return e;
+        }
+        p.graph_in = in;
+        p.graph_out = out;
+    }
+    return hipGraphLaunch(p.graph_exec, stream);
+}
+
+std::string format_schedules(const RaggedPlan& plan)
+{
+    std::ostringstream os;
+    for (const RaggedClassInfo& c : plan.classes) {
+        os << c.key << " ";
+        for (size_t i = 0; i < c.sched.radices.size(); i++) os << (i ? "x" : "") << c.sched.radices[i];
+        os << " " << c.sched.threads << " " << c.sched.tpb << "\n";
+    }
+    return os.str();
+}
+
+}  // namespace asdfft

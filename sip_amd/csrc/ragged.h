@@ -1,0 +1,119 @@
+// Ragged batches: many transforms of different lengths and kinds (C2C, R2C, C2R) run
+// as one persistent launch per register budget, driven by a device-side work table,
+// optionally replayed from a hipGraph.  The reference has no counterpart; its nearest
+// equivalent is one plan and one exec per distinct (type, length)
+// (core/fft/fft_api.cpp:1124-1190 per size, sip_pta/csrc/FFTMixCache.cpp:18-40 caching).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fft_kernel_args.h"
+#include "kernels.h"
+#include "twiddles.h"
+
+namespace asdfft {
+
+// Largest workgroup of the ragged kernel (its __launch_bounds__).
+constexpr int kRaggedMaxBlock = 512;
+
+// ---- kernel side (ragged_kernel.hip) --------------------------------------------------
+const void* ragged_kernel_fn(int emax);
+// Raises the dynamic LDS limit when needed and reports resident workgroups per CU.
+hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu);
+hipError_t launch_ragged(int emax, int64_t grid, int block, size_t lds_bytes, const RaggedArgs& args,
+                         const RaggedClass* classes, const RaggedTile* tiles, hipStream_t stream);
+
+// ---- host side (ragged.cpp) -------------------------------------------------------------
+enum class RaggedKind { C2C, R2C, C2R };
+
+struct RaggedEntry {
+    RaggedKind kind = RaggedKind::C2C;
+    bool inverse = false;
+    int64_t n = 0;           // transform length (real length for R2C / C2R)
+    int64_t count = 0;       // transforms stored back to back
+    int64_t in_offset = 0;   // bytes from the input base
+    int64_t out_offset = 0;  // bytes from the output base
+};
+
+// One class's schedule (what the autotuner varies).
+struct ClassSchedule {
+    std::vector<int> radices;  // pass order
+    int threads = 0;           // threads per transform (>= the minimum for EMAX)
+    int tpb = 0;               // transforms per tile (0: as many as the workgroup holds)
+};
+
+struct RaggedOptions {
+    bool graph = false;     // replay the launches from an instantiated hipGraph
+    bool autotune = false;  // time candidate schedules on the device at plan time
+    int block16 = 0;        // workgroup size of the EMAX-16 launch (0: automatic)
+    // schedules fixed by the caller or a previous tuning run, keyed by class key
+    std::map<std::string, ClassSchedule> fixed;
+};
+
+struct RaggedClassInfo {
+    std::string key;    // "c2c:f:4096", "r2c:i:1000", ...
+    int variant = 0;
+    int emax = 16;
+    int64_t n = 0;      // complex length the Stockham passes run on
+    int64_t nreal = 0;  // length given by the caller
+    int64_t transforms = 0;
+    ClassSchedule sched;
+    size_t lds = 0;
+    double bytes = 0;   // algorithmic bytes of all its transforms
+};
+
+struct RaggedLaunchPlan {
+    int emax = 16;
+    int block = 256;
+    size_t lds_bytes = 0;
+    int64_t grid = 0;
+    int64_t ntiles = 0;
+    std::vector<int> class_ids;  // indices into RaggedPlan::classes
+    std::vector<RaggedClass> dev_classes;
+    std::vector<RaggedTile> tiles;
+    RaggedClass* d_classes = nullptr;
+    RaggedTile* d_tiles = nullptr;
+};
+
+struct RaggedPlan {
+    std::vector<RaggedEntry> entries;
+    std::vector<RaggedClassInfo> classes;
+    std::vector<RaggedLaunchPlan> launches;
+    std::vector<int64_t> offsets;  // host copy of the offset table
+    int64_t* d_offsets = nullptr;
+    int* d_queue = nullptr;        // two words per launch, 128 bytes apart
+    std::vector<TablePtr> tables;
+    int device = -1;
+    bool ready = false;            // device tables uploaded
+    size_t in_extent = 0, out_extent = 0;  // bytes touched past each base
+    bool all_inplace_c2c = false;
+    double bytes = 0;
+    double flops = 0;              // 5 N log2 N per complex-equivalent transform
+    RaggedOptions opt;
+    // graph replay state
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<hipGraphNode_t> nodes;
+    const void* graph_in = nullptr;
+    void* graph_out = nullptr;
+    std::string describe;
+    ~RaggedPlan();
+};
+
+// Validates the entries and builds the plan ("" on success, otherwise the reason).
+// `unsupported` is set when the entries are valid but outside what the kernels handle.
+std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOptions& opt, RaggedPlan* plan,
+                         bool* unsupported);
+
+// Enqueues the batch on `stream` (graph replay when the plan was built with graph = true).
+hipError_t exec_ragged(RaggedPlan& plan, const void* in_base, void* out_base, hipStream_t stream);
+
+// Text form of a tuned schedule map (one class per line) and its parser.
+std::string format_schedules(const RaggedPlan& plan);
+
+}  // namespace asdfft

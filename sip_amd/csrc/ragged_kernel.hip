@@ -1,0 +1,175 @@
+// Persistent ragged-batch kernel: one launch transforms tiles of different lengths and
+// kinds (C2C, even/odd R2C and C2R rows) taken from a device-side work table.
+//
+// Work table: tiles[k] = {class, transforms in the tile, first offset-table entry};
+// classes[c] holds the Stockham pass list and geometry of one (kind, direction, length).
+// Workgroups take tile indices from queue[0] with an atomic increment until the table is
+// exhausted (tiles are sorted by decreasing cost on the host, so the largest go first and
+// the small ones fill the tail).  The last workgroup to leave sets both queue words back
+// to zero, so the next launch (or graph replay) starts from an empty queue without a
+// memset between launches.
+#include "ragged.h"
+#include "stockham_impl.h"
+
+namespace asdfft {
+
+namespace {
+
+// Lanes past tpb * T of a fixed-size workgroup own no LDS block; this thread index lies
+// past every pass's butterfly range, so they issue no loads, LDS accesses or stores but
+// still reach every barrier.
+constexpr int kIdleThread = 1 << 20;
+
+template <int EMAX, int PRO, int EPI>
+__device__ __forceinline__ void ragged_tile(const RaggedArgs& r, const FftArgs& a, int count, int64_t first,
+                                            float* smem)
+{
+    const Geo g{__builtin_amdgcn_readfirstlane(a.n), __builtin_amdgcn_readfirstlane(a.threads),
+                __builtin_amdgcn_readfirstlane(a.tpb), __builtin_amdgcn_readfirstlane(a.lds_tstride)};
+    float* lre = smem;
+    float* lim = smem + __builtin_amdgcn_readfirstlane(a.lds_plane);
+    int tt, tid;
+    lane_map<kMapRows>(g, tt, tid);
+    const bool lane_ok = tt < g.tpb;
+    const bool live = lane_ok && tt < count;
+    if (!lane_ok) {
+        tt = 0;
+        tid = kIdleThread;
+    }
+    Io io;
+    io.in = r.in_base;
+    io.out = r.out_base;
+    io.col = 0;
+    io.cin = __builtin_amdgcn_readfirstlane(a.conj_in) ? 0x80000000u : 0u;
+    io.cout = __builtin_amdgcn_readfirstlane(a.conj_out) ? 0x80000000u : 0u;
+    io.in_stride = 1;
+    io.out_stride = 1;
+    if (live) {
+        const int64_t* o = r.offsets + 2 * (first + tt);
+        io.in = static_cast<const char*>(r.in_base) + o[0];
+        io.out = static_cast<char*>(r.out_base) + o[1];
+    }
+    const int passes = __builtin_amdgcn_readfirstlane(a.passes);
+    for (int p = 0; p < passes; ++p) {
+        switch (__builtin_amdgcn_readfirstlane(a.pass[p].radix)) {
+#define ASDFFT_CASE(RR) \
+    case RR: maybe_pass<EMAX, RR, kMapRows, PRO, EPI>(a, g, p, lre, lim, tt, tid, live, io); break;
+            ASDFFT_CASE(16)
+            ASDFFT_CASE(12)
+            ASDFFT_CASE(10)
+            ASDFFT_CASE(9)
+            ASDFFT_CASE(8)
+            ASDFFT_CASE(6)
+            ASDFFT_CASE(4)
+            ASDFFT_CASE(2)
+            ASDFFT_CASE(19)
+            ASDFFT_CASE(17)
+            ASDFFT_CASE(13)
+            ASDFFT_CASE(11)
+            ASDFFT_CASE(7)
+            ASDFFT_CASE(5)
+            ASDFFT_CASE(3)
+#undef ASDFFT_CASE
+            default:
+                break;
+        }
+    }
+    lds_epilogue<kMapRows, EPI>(a, g, lre, lim, tt, tid, 0, live, io);
+}
+
+template <int EMAX>
+__global__ void __launch_bounds__(kRaggedMaxBlock) __attribute__((amdgpu_waves_per_eu(min_waves<EMAX>())))
+ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const RaggedTile* __restrict__ tiles)
+{
+    extern __shared__ float smem[];
+    __shared__ int s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&r.queue[0], 1);
+        lds_barrier();
+        const int k = __builtin_amdgcn_readfirstlane(s_tile);
+        lds_barrier();  // every wave has read s_tile before thread 0 takes the next index
+        if (k >= r.ntiles) break;
+        const int cls = __builtin_amdgcn_readfirstlane(tiles[k].cls);
+        const int count = __builtin_amdgcn_readfirstlane(tiles[k].count);
+        const int64_t first = tiles[k].first;
+        const RaggedClass& c = classes[cls];
+        switch (__builtin_amdgcn_readfirstlane(c.variant)) {
+            case kVarRowsC2C:
+                ragged_tile<EMAX, kProComplex, kEpiComplex>(r, c.a, count, first, smem);
+                break;
+            case kVarR2CEven:
+                ragged_tile<EMAX, kProComplex, kEpiR2CEven>(r, c.a, count, first, smem);
+                break;
+            case kVarC2REven:
+                ragged_tile<EMAX, kProC2REven, kEpiComplex>(r, c.a, count, first, smem);
+                break;
+            case kVarR2COdd:
+                ragged_tile<EMAX, kProReal, kEpiHalf>(r, c.a, count, first, smem);
+                break;
+            case kVarC2ROdd:
+                ragged_tile<EMAX, kProHerm, kEpiReal>(r, c.a, count, first, smem);
+                break;
+            default:
+                break;
+        }
+        // the next tile's first LDS write sits behind its first exchange barrier, which
+        // every wave reaches only after finishing this tile's LDS reads
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int done = atomicAdd(&r.queue[1], 1);
+        if (done == (int)gridDim.x - 1) {
+            // every other workgroup has taken its last index: reset for the next launch
+            atomicExch(&r.queue[0], 0);
+            atomicExch(&r.queue[1], 0);
+        }
+    }
+}
+
+template <int EMAX>
+const void* kernel_fn()
+{
+    return reinterpret_cast<const void*>(&ragged_kernel<EMAX>);
+}
+
+}  // namespace
+
+const void* ragged_kernel_fn(int emax)
+{
+    return emax <= 16 ? kernel_fn<16>() : kernel_fn<32>();
+}
+
+hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu)
+{
+    const void* fn = ragged_kernel_fn(emax);
+    if (lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    int n = 0;
+    hipError_t e;
+    if (emax <= 16) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<16>, block, lds_bytes);
+    } else {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<32>, block, lds_bytes);
+    }
+    if (e != hipSuccess) return e;
+    *per_cu = n;
+    return hipSuccess;
+}
+
+hipError_t launch_ragged(int emax, int64_t grid, int block, size_t lds_bytes, const RaggedArgs& args,
+                         const RaggedClass* classes, const RaggedTile* tiles, hipStream_t stream)
+{
+    if (block <= 0 || block > kRaggedMaxBlock || block % 64 != 0 || grid <= 0) return hipErrorInvalidValue;
+    if (emax <= 16) {
+        hipLaunchKernelGGL(ragged_kernel<16>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
+                           classes, tiles);
+    } else {
+        hipLaunchKernelGGL(ragged_kernel<32>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
+                           classes, tiles);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace asdfft

@@ -137,6 +137,113 @@ class Plan:
             pass
 
 
+def _side_bytes(kind: str, n: int):
+    """(input bytes, output bytes) of one transform of this kind and length."""
+    h = n // 2 + 1
+    return {"c2c": (8 * n, 8 * n), "r2c": (4 * n, 8 * h), "c2r": (8 * h, 4 * n)}[kind]
+
+
+class RaggedPlan:
+    """A ragged batch (asdFftExtMakePlanRagged): entries of different kinds and lengths
+    transformed by one exec over a packed input buffer and a packed output buffer.
+
+    entries: (kind, n, count, inverse) tuples, optionally followed by explicit byte
+    offsets (in_offset, out_offset); without them the entries are packed back to back in
+    order, each starting on an 8-byte boundary.
+    """
+
+    def __init__(self, entries, graph: bool = False, autotune: bool = False):
+        self.L = N.lib()
+        rows = []
+        ioff = ooff = 0
+        for e in entries:
+            kind, n, count, inverse = e[0], int(e[1]), int(e[2]), bool(e[3])
+            ib, ob = _side_bytes(kind, n)
+            if len(e) >= 6:
+                i0, o0 = int(e[4]), int(e[5])
+            else:
+                i0, o0 = ioff, ooff
+                ioff = (ioff + ib * count + 7) // 8 * 8
+                ooff = (ooff + ob * count + 7) // 8 * 8
+            rows.append((kind, n, count, inverse, i0, o0))
+        self.entries = rows
+        arr = (N.RaggedEntry * len(rows))()
+        for i, (kind, n, count, inverse, i0, o0) in enumerate(rows):
+            arr[i].type = _TYPE[kind]
+            arr[i].direction = N.ASCEND_FFT_INVERSE if inverse else N.ASCEND_FFT_FORWARD
+            arr[i].n = n
+            arr[i].count = count
+            arr[i].in_offset = i0
+            arr[i].out_offset = o0
+        h = ctypes.c_void_p()
+        N.check("asdFftCreate", self.L.asdFftCreate(ctypes.byref(h)))
+        self.handle = h
+        flags = (N.ASDFFT_RAGGED_GRAPH if graph else 0) | (N.ASDFFT_RAGGED_AUTOTUNE if autotune else 0)
+        try:
+            N.check("asdFftExtMakePlanRagged", self.L.asdFftExtMakePlanRagged(h, arr, len(rows), flags))
+        except Exception:
+            self.L.asdFftDestroy(h)
+            self.handle = None
+            raise
+        ie, oe = ctypes.c_int64(0), ctypes.c_int64(0)
+        by, fl = ctypes.c_double(0), ctypes.c_double(0)
+        N.check("asdFftExtRaggedInfo", self.L.asdFftExtRaggedInfo(h, ctypes.byref(ie), ctypes.byref(oe),
+                                                                   ctypes.byref(by), ctypes.byref(fl)))
+        self.in_extent, self.out_extent = ie.value, oe.value
+        self.algorithmic_bytes, self.flops = by.value, fl.value
+        self._stream = None
+
+    def describe(self) -> str:
+        need = ctypes.c_size_t(0)
+        self.L.asdFftExtDescribePlan(self.handle, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(need.value + 1)
+        self.L.asdFftExtDescribePlan(self.handle, buf, len(buf), None)
+        return buf.value.decode()
+
+    def views(self, buf, side: str):
+        """Per-entry tensor views into a packed uint8 buffer: side 'in' or 'out'."""
+        import torch
+        out = []
+        for kind, n, count, _, i0, o0 in self.entries:
+            h = n // 2 + 1
+            real = (kind == "r2c" and side == "in") or (kind == "c2r" and side == "out")
+            width = n if (kind == "c2c" or real) else h
+            dt = torch.float32 if real else torch.complex64
+            esz = 4 if real else 8
+            off = i0 if side == "in" else o0
+            out.append(buf[off:off + count * width * esz].view(dt).view(count, width))
+        return out
+
+    def execute(self, in_buf, out_buf, stream_handle: int | None = None):
+        """Enqueues the batch; in_buf / out_buf are device tensors covering the extents."""
+        import torch
+        if in_buf.numel() * in_buf.element_size() < self.in_extent:
+            raise ValueError("input buffer smaller than the plan's input extent")
+        if out_buf.numel() * out_buf.element_size() < self.out_extent:
+            raise ValueError("output buffer smaller than the plan's output extent")
+        s = stream_handle if stream_handle is not None else torch.cuda.current_stream(in_buf.device).cuda_stream
+        if s and s != self._stream:
+            N.check("asdFftSetStream", self.L.asdFftSetStream(self.handle, ctypes.c_void_p(s)))
+            self._stream = s
+        N.check("asdFftExtExecRagged", self.L.asdFftExtExecRagged(self.handle, ctypes.c_void_p(in_buf.data_ptr()),
+                                                                   ctypes.c_void_p(out_buf.data_ptr())))
+        return out_buf
+
+    def synchronize(self):
+        N.check("asdFftSynchronize", self.L.asdFftSynchronize(self.handle))
+
+    def close(self):
+        if self.handle is not None:
+            self.L.asdFftDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PlanCache:
     """LRU of plans keyed by (kind, dims, batch, inverse); the reference keeps 10."""
 
