@@ -4,6 +4,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <sstream>
 
@@ -550,6 +551,51 @@ std::vector<int> choose_radices(int64_t n)
     std::sort(r.begin(), r.end(), [](int x, int y) { return x > y; });
     if ((int)r.size() > kMaxPasses) return {};
     return r;
+}
+
+std::vector<std::vector<int>> radix_multisets(int64_t n, int extra, int limit)
+{
+    const std::vector<int> best = choose_radices(n);
+    if (best.empty()) return {};
+    const int max_passes = std::min<int>(kMaxPasses, (int)best.size() + extra);
+    std::vector<const RadixInfo*> order;
+    for (const RadixInfo& r : kRadices) order.push_back(&r);
+    std::sort(order.begin(), order.end(), [](const RadixInfo* a, const RadixInfo* b) { return a->radix > b->radix; });
+    struct Cand {
+        std::vector<int> radices;
+        double cost;
+        int maxr;
+    };
+    std::vector<Cand> found;
+    std::vector<int> cur;
+    // non-increasing radix sequences: each factor is taken from position `start` on
+    std::function<void(int64_t, size_t, double)> dfs = [&](int64_t m, size_t start, double cost) {
+        if (m == 1) {
+            found.push_back({cur, cost, cur.empty() ? 0 : cur.front()});
+            return;
+        }
+        if ((int)cur.size() >= max_passes) return;
+        for (size_t i = start; i < order.size(); i++) {
+            const int r = order[i]->radix;
+            if (m % r != 0) continue;
+            cur.push_back(r);
+            dfs(m / r, i, cost + order[i]->cost);
+            cur.pop_back();
+        }
+    };
+    dfs(n, 0, 0.0);
+    std::sort(found.begin(), found.end(), [](const Cand& a, const Cand& b) {
+        if (a.radices.size() != b.radices.size()) return a.radices.size() < b.radices.size();
+        const bool a16 = a.maxr <= 16, b16 = b.maxr <= 16;
+        if (a16 != b16) return a16;
+        return a.cost < b.cost;
+    });
+    std::vector<std::vector<int>> out;
+    for (const Cand& c : found) {
+        if ((int)out.size() >= limit) break;
+        out.push_back(c.radices);
+    }
+    return out;
 }
 
 int threads_for(int64_t n, const std::vector<int>& radices, int emax)

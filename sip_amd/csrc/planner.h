@@ -37,6 +37,11 @@ struct Schedule {
 // when n has a prime factor outside {2,3,5,7,11,13,17,19} or n == 1.
 std::vector<int> choose_radices(int64_t n);
 
+// Radix multisets (listed in descending order) whose product is n, with at most
+// `extra` passes more than the fewest possible; sorted by passes, then by whether
+// every radix is <= 16, then by butterfly cost.  At most `limit` of them.
+std::vector<std::vector<int>> radix_multisets(int64_t n, int extra, int limit);
+
 // Threads per transform needed to cover every pass with EMAX/R butterflies per
 // thread, or 0 when this EMAX cannot hold a radix of the sequence or the
 // transform does not fit the EMAX's workgroup limit.

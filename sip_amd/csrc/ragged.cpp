@@ -12,6 +12,8 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
 #include <sstream>
 
 #include "planner.h"
@@ -164,8 +166,10 @@ RaggedPlan::~RaggedPlan()
     if (have && prev != device) (void)hipSetDevice(prev);
 }
 
-std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOptions& opt, RaggedPlan* plan,
-                         bool* unsupported)
+namespace {
+
+std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOptions& opt, RaggedPlan* plan,
+                       bool* unsupported)
 {
     *unsupported = false;
     plan->entries = entries;
@@ -205,6 +209,8 @@ std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOp
             class_of[key] = (int)plan->classes.size();
             RaggedClassInfo c;
             c.key = key;
+            c.kind = e.kind;
+            c.inverse = e.inverse;
             c.variant = s.variant;
             c.n = s.n;
             c.nreal = e.n;
@@ -270,7 +276,8 @@ std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOp
             one = std::max(one, class_lds(plan->classes[ci], 1));
         }
         if (L.class_ids.empty()) continue;
-        int block = (emax == 16 && opt.block16 > 0) ? opt.block16 : 512;
+        const int want = (emax == 16) ? opt.block16 : opt.block32;
+        int block = want > 0 ? want : 512;
         block = std::max(block, (maxT + 63) / 64 * 64);
         block = std::min(block, kRaggedMaxBlock);
         L.block = block;
@@ -411,8 +418,6 @@ std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOp
     return "";
 }
 
-namespace {
-
 RaggedArgs args_for(const RaggedPlan& p, size_t li, const void* in, void* out)
 {
     RaggedArgs a{};
@@ -500,6 +505,220 @@ std::string format_schedules(const RaggedPlan& plan)
         os << " " << c.sched.threads << " " << c.sched.tpb << "\n";
     }
     return os.str();
+}
+
+
+// ---- autotuner -----------------------------------------------------------------------
+namespace {
+
+constexpr double kTuneBytes = 8.0 * 1024 * 1024;  // data per timed single-class launch
+constexpr int kTuneReps = 4;
+constexpr int kTuneCandidates = 8;
+
+std::mutex g_tune_mu;
+std::map<std::string, ClassSchedule>& tune_cache()
+{
+    static auto* m = new std::map<std::string, ClassSchedule>();
+    return *m;
+}
+
+struct Scratch {
+    void* in = nullptr;
+    void* out = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t need)
+    {
+        if (need <= bytes) return true;
+        release();
+        if (hipMalloc(&in, need) != hipSuccess || hipMalloc(&out, need) != hipSuccess) {
+            release();
+            return false;
+        }
+        if (hipMemset(in, 0, need) != hipSuccess) return false;
+        bytes = need;
+        return true;
+    }
+    void release()
+    {
+        free_device(in);
+        free_device(out);
+        in = out = nullptr;
+        bytes = 0;
+    }
+    ~Scratch() { release(); }
+};
+
+// Average event-timed duration of one exec (after one untimed exec), +inf on failure.
+double time_exec(RaggedPlan& p, void* in, void* out, hipStream_t s, int reps)
+{
+    const double inf = 1e30;
+    if (exec_ragged(p, in, out, s) != hipSuccess) return inf;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return inf;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return inf;
+    }
+    double ms = inf;
+    bool ok = hipEventRecord(e0, s) == hipSuccess;
+    for (int r = 0; ok && r < reps; r++) ok = exec_ragged(p, in, out, s) == hipSuccess;
+    ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float f = 0;
+    if (ok && hipEventElapsedTime(&f, e0, e1) == hipSuccess) ms = (double)f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+std::string cache_key(int device, const std::string& key, const RaggedOptions& opt)
+{
+    return std::to_string(device) + "|" + key + "|" + std::to_string(opt.block16) + "|" + std::to_string(opt.block32);
+}
+
+// Times every candidate schedule of class c (alone, replicated to about kTuneBytes) and
+// returns the fastest.  Falls back to the default schedule when nothing could be timed.
+ClassSchedule tune_class(const RaggedClassInfo& c, const RaggedOptions& base, Scratch& scratch, hipStream_t s,
+                         int* timed)
+{
+    RaggedEntry e;
+    e.kind = c.kind;
+    e.inverse = c.inverse;
+    e.n = c.nreal;
+    const int64_t ib = in_bytes_of(e), ob = out_bytes_of(e);
+    const int64_t want = (int64_t)(kTuneBytes / (double)(ib + ob)) + 1;
+    e.count = std::max<int64_t>(c.transforms, std::min<int64_t>(want, 1 << 16));
+    const size_t need = (size_t)std::max(ib, ob) * (size_t)e.count;
+    ClassSchedule best = c.sched;
+    best.tpb = 0;
+    if (!scratch.ensure(need)) return best;
+    double best_ms = 1e30;
+    for (const ClassSchedule& cand : class_candidates(c.n, kTuneCandidates)) {
+        RaggedOptions o = base;
+        o.graph = false;
+        o.autotune = false;
+        o.fixed.clear();
+        o.fixed[c.key] = cand;
+        RaggedPlan p;
+        bool unsup = false;
+        if (!build_once({e}, o, &p, &unsup).empty() || !p.ready) continue;
+        const double ms = time_exec(p, scratch.in, scratch.out, s, kTuneReps);
+        (*timed)++;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cand;
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+std::vector<ClassSchedule> class_candidates(int64_t n, int limit)
+{
+    std::vector<ClassSchedule> out;
+    auto add = [&](const std::vector<int>& r, int threads) {
+        for (const ClassSchedule& c : out) {
+            if (c.radices == r && c.threads == threads) return;
+        }
+        ClassSchedule c;
+        c.radices = r;
+        c.threads = threads;
+        out.push_back(c);
+    };
+    for (const std::vector<int>& ms : radix_multisets(n, 1, 4)) {
+        std::vector<int> asc(ms.rbegin(), ms.rend());
+        for (const std::vector<int>& order : {ms, asc}) {
+            int t0 = 0;
+            for (int emax : {16, 32}) {
+                const int t = min_threads(n, order, emax);
+                if (t > 0 && t <= kRaggedMaxBlock) {
+                    t0 = t;
+                    break;
+                }
+            }
+            if (t0 == 0) continue;
+            add(order, t0);
+            if (2 * t0 <= kRaggedMaxBlock && 2 * t0 <= n) add(order, 2 * t0);
+        }
+    }
+    // the default schedule first, then the rest in enumeration order
+    if ((int)out.size() > limit) out.resize(limit);
+    return out;
+}
+
+void clear_tuning_cache()
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    tune_cache().clear();
+}
+
+std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOptions& opt, RaggedPlan* plan,
+                         bool* unsupported)
+{
+    if (!opt.autotune || !device_available()) return build_once(entries, opt, plan, unsupported);
+    const auto t0 = std::chrono::steady_clock::now();
+    // validate and learn the classes with the default schedules first
+    RaggedOptions plain = opt;
+    plain.autotune = false;
+    plain.graph = false;
+    RaggedPlan probe;
+    std::string why = build_once(entries, plain, &probe, unsupported);
+    if (!why.empty() || !probe.ready) return why.empty() ? build_once(entries, opt, plan, unsupported) : why;
+    int device = 0;
+    (void)hipGetDevice(&device);
+    hipStream_t s = nullptr;
+    if (hipStreamCreate(&s) != hipSuccess) return build_once(entries, plain, plan, unsupported);
+    int timed = 0;
+    RaggedOptions tuned = plain;
+    {
+        Scratch scratch;
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        for (const RaggedClassInfo& c : probe.classes) {
+            if (opt.fixed.count(c.key)) continue;
+            const std::string ck = cache_key(device, c.key, plain);
+            auto it = tune_cache().find(ck);
+            if (it == tune_cache().end()) it = tune_cache().emplace(ck, tune_class(c, plain, scratch, s, &timed)).first;
+            tuned.fixed[c.key] = it->second;
+        }
+        // workgroup size of each launch, timed on the whole batch with the tuned schedules
+        if (opt.block16 == 0 || opt.block32 == 0) {
+            size_t need = std::max(probe.in_extent, probe.out_extent);
+            if (scratch.ensure(need)) {
+                double best = 1e30;
+                int b16 = 512, b32 = 512;
+                for (int cand : {256, 512}) {
+                    RaggedOptions o = tuned;
+                    if (opt.block16 == 0) o.block16 = cand;
+                    if (opt.block32 == 0) o.block32 = 512;
+                    RaggedPlan p;
+                    bool unsup = false;
+                    if (!build_once(entries, o, &p, &unsup).empty() || !p.ready) continue;
+                    const double ms = time_exec(p, scratch.in, scratch.out, s, kTuneReps);
+                    timed++;
+                    if (ms < best) {
+                        best = ms;
+                        b16 = o.block16;
+                        b32 = o.block32;
+                    }
+                }
+                if (opt.block16 == 0) tuned.block16 = b16;
+                if (opt.block32 == 0) tuned.block32 = b32;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    tuned.graph = opt.graph;
+    tuned.autotune = false;
+    why = build_once(entries, tuned, plan, unsupported);
+    plan->opt.autotune = true;
+    plan->tune_timed = timed;
+    plan->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::ostringstream os;
+    os << plan->describe << "  autotuned: " << timed << " timings in " << (int)plan->tune_ms << " ms, block16="
+       << tuned.block16 << "\n";
+    plan->describe = os.str();
+    return why;
 }
 
 }  // namespace asdfft

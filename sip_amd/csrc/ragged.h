@@ -51,12 +51,15 @@ struct RaggedOptions {
     bool graph = false;     // replay the launches from an instantiated hipGraph
     bool autotune = false;  // time candidate schedules on the device at plan time
     int block16 = 0;        // workgroup size of the EMAX-16 launch (0: automatic)
+    int block32 = 0;        // workgroup size of the EMAX-32 launch (0: automatic)
     // schedules fixed by the caller or a previous tuning run, keyed by class key
     std::map<std::string, ClassSchedule> fixed;
 };
 
 struct RaggedClassInfo {
     std::string key;    // "c2c:f:4096", "r2c:i:1000", ...
+    RaggedKind kind = RaggedKind::C2C;
+    bool inverse = false;
     int variant = 0;
     int emax = 16;
     int64_t n = 0;      // complex length the Stockham passes run on
@@ -82,6 +85,8 @@ struct RaggedLaunchPlan {
 
 struct RaggedPlan {
     std::vector<RaggedEntry> entries;
+    double tune_ms = 0;            // plan-time autotuning wall time (0 when not tuned)
+    int tune_timed = 0;            // candidate schedules timed
     std::vector<RaggedClassInfo> classes;
     std::vector<RaggedLaunchPlan> launches;
     std::vector<int64_t> offsets;  // host copy of the offset table
@@ -113,7 +118,15 @@ std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOp
 // Enqueues the batch on `stream` (graph replay when the plan was built with graph = true).
 hipError_t exec_ragged(RaggedPlan& plan, const void* in_base, void* out_base, hipStream_t stream);
 
-// Text form of a tuned schedule map (one class per line) and its parser.
+// One line per class: key, radix order, threads per transform, transforms per tile.
 std::string format_schedules(const RaggedPlan& plan);
+
+// Candidate schedules the autotuner times for one class (radix multisets within one
+// pass of the fewest, in descending and ascending order, at the minimum and at twice
+// the minimum threads per transform).
+std::vector<ClassSchedule> class_candidates(int64_t n, int limit);
+
+// Drops the process-wide cache of tuned schedules (measurement and tests).
+void clear_tuning_cache();
 
 }  // namespace asdfft

@@ -227,3 +227,20 @@ def test_ragged_workload_full_size(torch_dev, oracle):
         print(f"ragged workload: {len(work)} transforms, {len(groups)} classes, worst err/tol {worst:.3f}")
     finally:
         plan.close()
+
+
+def test_autotuned_plan_matches_oracle(torch_dev, oracle):
+    """Schedules picked by the plan-time autotuner (other radix orders, more threads per
+    transform, another workgroup size) still meet the accuracy bound."""
+    from sip_amd.fft import RaggedPlan
+    rng = np.random.default_rng(17)
+    spec = [("c2c", 4096, 6, False), ("c2c", 6000, 3, True), ("r2c", 8192, 4, False), ("c2r", 1000, 5, True),
+            ("c2c", 16384, 2, False), ("r2c", 243, 9, False), ("c2c", 96, 30, True)]
+    plan = RaggedPlan(spec, autotune=True)
+    try:
+        assert "autotuned:" in plan.describe()
+        inputs = [make_input(rng, k, n, c) for k, n, c, _ in spec]
+        outs = run_ragged(torch_dev, plan, inputs)
+        check_entries(oracle, plan.entries, inputs, outs)
+    finally:
+        plan.close()
