@@ -75,6 +75,60 @@ int32_t asdFftExtExecRagged(void* handle, const void* in_base, void* out_base);
  * flop count 5 n log2 n summed over the transforms. */
 int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t* out_extent, double* bytes, double* flops);
 
+/* ---- serving loop ------------------------------------------------------------------------
+ * A server admits FFT requests in arrival order while their input + output bytes (plus a
+ * plan workspace) fit an HBM budget, and runs them in batches: whenever one of `slots`
+ * in-flight batches completes, everything admitted meanwhile forms the next batch.  1D
+ * requests of lengths the ragged kernel handles run as one persistent launch per batch
+ * (class descriptors and twiddle tables stay resident); other requests (2D, longer 1D, odd
+ * factors) run through cached plans on the batch's stream.  Each slot has its own stream,
+ * so batches overlap on the device.  With batching = 0 every request is its own launch
+ * group through the plain plan path.  Data pointers are device pointers the caller owns
+ * until the request completes.  No reference counterpart: the reference's callers make
+ * one plan and one exec per request (sip_pta/csrc/FFTMixCache.cpp:18-40). */
+typedef struct asdFftExtServeConfig {
+    uint64_t hbm_budget_bytes;   /* 0: 8 GiB */
+    int32_t slots;               /* batches in flight, 1..8 (0: 3) */
+    int32_t max_batch_requests;  /* requests per batch (0: 256) */
+    int32_t batching;            /* 1: continuous batching, 0: one plain exec per request */
+    int32_t reserved;
+} asdFftExtServeConfig;
+
+typedef struct asdFftExtServeRequest {
+    int32_t type;       /* asdFftType value: C2C 0x10, C2R 0x11, R2C 0x12 */
+    int32_t direction;  /* asdFftDirection value */
+    int32_t rank;       /* 1 or 2 */
+    int32_t reserved;
+    int64_t dims[2];    /* n, or (X, Y); real length on the last axis for R2C / C2R */
+    int64_t batch;
+    const void* in;
+    void* out;
+} asdFftExtServeRequest;
+
+typedef struct asdFftExtServeCounters {
+    uint64_t submitted, completed, batches;
+    uint64_t ragged_requests, plain_requests;
+    uint64_t peak_inflight_bytes, resident_bytes;
+} asdFftExtServeCounters;
+
+/* No reference counterpart (serving form).  Creates a server on the current device. */
+int32_t asdFftExtServeCreate(void** server, const asdFftExtServeConfig* config);
+/* Queues a request; *ticket identifies it.  ACL_ERROR_INVALID_PARAM for malformed requests
+ * or one larger than the budget, ACL_ERROR_API_NOT_SUPPORT for sizes no plan handles. */
+int32_t asdFftExtServeSubmit(void* server, const asdFftExtServeRequest* request, uint64_t* ticket);
+/* Waits for a ticket (timeout_us < 0: no limit) and retires it; *latency_us = submission to
+ * completion.  ACL_ERROR_INVALID_PARAM for an unknown ticket, ACL_ERROR_INTERNAL_ERROR if
+ * its batch failed or on timeout (the ticket then stays valid). */
+int32_t asdFftExtServeWait(void* server, uint64_t ticket, int64_t timeout_us, double* latency_us);
+/* Submits requests[i] at arrival_us[i] microseconds after the call from the calling thread,
+ * waits for all of them and writes each latency measured from its scheduled arrival (-1
+ * for a request that was refused). */
+int32_t asdFftExtServeReplay(void* server, const asdFftExtServeRequest* requests, const double* arrival_us,
+                             int64_t count, double* latency_us);
+int32_t asdFftExtServeStats(void* server, asdFftExtServeCounters* stats);
+/* Waits for every queued request, then releases the server. */
+int32_t asdFftExtServeDestroy(void* server);
+
 #ifdef __cplusplus
 }
 #endif

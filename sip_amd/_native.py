@@ -64,6 +64,38 @@ class FftError(RuntimeError):
 
 _lib = None
 
+
+class ServeConfig(ctypes.Structure):
+    """asdFftExtServeConfig (include/asdfft_ext.h)."""
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("slots", ctypes.c_int32),
+        ("max_batch_requests", ctypes.c_int32),
+        ("batching", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ServeRequest(ctypes.Structure):
+    """asdFftExtServeRequest (include/asdfft_ext.h)."""
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("direction", ctypes.c_int32),
+        ("rank", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("dims", ctypes.c_int64 * 2),
+        ("batch", ctypes.c_int64),
+        ("input", ctypes.c_void_p),
+        ("output", ctypes.c_void_p),
+    ]
+
+
+class ServeCounters(ctypes.Structure):
+    """asdFftExtServeCounters (include/asdfft_ext.h)."""
+    _fields_ = [(name, ctypes.c_uint64) for name in
+                ("submitted", "completed", "batches", "ragged_requests", "plain_requests", "peak_inflight_bytes",
+                 "resident_bytes")]
+
 ASDFFT_RAGGED_GRAPH = 0x1
 ASDFFT_RAGGED_AUTOTUNE = 0x2
 
@@ -116,6 +148,13 @@ SIGNATURES = [
     ("asdFftExtExecRagged", _I32, [_VP, _VP, _VP]),
     ("asdFftExtRaggedInfo", _I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)]),
+    ("asdFftExtServeCreate", _I32, [ctypes.POINTER(_VP), ctypes.POINTER(ServeConfig)]),
+    ("asdFftExtServeSubmit", _I32, [_VP, ctypes.POINTER(ServeRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    ("asdFftExtServeWait", _I32, [_VP, ctypes.c_uint64, _I64, ctypes.POINTER(ctypes.c_double)]),
+    ("asdFftExtServeReplay", _I32, [_VP, ctypes.POINTER(ServeRequest), ctypes.POINTER(ctypes.c_double), _I64,
+                                    ctypes.POINTER(ctypes.c_double)]),
+    ("asdFftExtServeStats", _I32, [_VP, ctypes.POINTER(ServeCounters)]),
+    ("asdFftExtServeDestroy", _I32, [_VP]),
 ]
 
 

@@ -33,7 +33,8 @@ HIPCC_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
               f"-I{ROCM}/include", f"-I{INCLUDE}", f"-I{CSRC}"]
-HOST_SOURCES = ["acl_tensor.cpp", "twiddles.cpp", "planner.cpp", "ragged.cpp", "fft_api.cpp"]
+HOST_SOURCES = ["acl_tensor.cpp", "twiddles.cpp", "planner.cpp", "ragged.cpp", "server.cpp", "serve_api.cpp",
+                "fft_api.cpp"]
 KERNEL_HEADERS = ["stockham_impl.h", "fft_kernel_args.h", "kernels.h", "radix_consts.h", "ragged.h"]
 # hand-written kernel translation units besides the generated instances
 KERNEL_SOURCES = ["dft_direct.hip", "ragged_kernel.hip"]
@@ -55,20 +56,6 @@ def static_instances():
         v = [int(x) for x in line.split()]
         out.append((v[0], v[1], v[2], v[3], v[4], tuple(v[5:])))
     return out
-
-
-def twiddle_lds_bytes(variant, emax, radices):
-    """Mirror of twiddles_in_lds() in stockham_impl.h: persistent prefetching row kernels
-    (variants with plain complex/real loads, EMAX 16) stage tables of <= 48 KiB in LDS."""
-    rows_plain = variant in (0, 2, 4, 7)  # kMapRows with kProComplex / kProReal
-    if not rows_plain or emax > 16:
-        return 0
-    entries, span = 0, 1
-    for r in radices:
-        if span > 1:
-            entries += span * (r - 1)
-        span *= r
-    return entries * 8 if entries * 8 <= 48 * 1024 else 0
 
 
 def generate_static_sources():
@@ -99,9 +86,9 @@ def generate_static_sources():
     reg.append("const StaticEntry kStaticInstances[] = {")
     for k, (v, e, n, t, tpb, rad) in enumerate(entries):
         reg.append(f"    {{{v}, {e}, {n}, {t}, {tpb}, {len(rad)}, {{{', '.join(map(str, rad))}}}, "
-                   f"&launch_static_entry_{k}, {twiddle_lds_bytes(v, e, rad)}}},")
+                   f"&launch_static_entry_{k}}},")
     if not entries:
-        reg.append("    {0, 0, 0, 0, 0, 0, {0}, nullptr, 0},")
+        reg.append("    {0, 0, 0, 0, 0, 0, {0}, nullptr},")
     reg.append("};")
     reg.append(f"const int kNumStaticInstances = {len(entries)};")
     reg.append("}  // namespace asdfft")

@@ -21,16 +21,18 @@ enum Variant : int {
     kVarDirect = 8,  // dft_direct_kernel
 };
 
+// Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a
+// workgroup.  The limit only admits launches, so it is never lowered to a launch's own size
+// (launches of different sizes from several threads could otherwise find it lowered).
+constexpr int kMaxDynamicLds = 160 * 1024;
+
 struct LaunchShape {
     int static_id = -1;  // index into kStaticInstances, or -1 for the run-time kernel
     int variant;
     int emax;          // registers per thread (complex points): 16 or 32
     int block;         // threads per workgroup
     size_t lds_bytes;  // dynamic LDS
-    int64_t grid;      // workgroups (tiles of tpb transforms)
-    // specialised kernels are persistent: each workgroup runs at least this many tiles
-    // (more when the tiles do not all fit on the device at once)
-    int tiles_per_block = 1;
+    int64_t grid;      // workgroups
 };
 
 hipError_t launch_stockham(const LaunchShape& s, const FftArgs& args, hipStream_t stream);
@@ -40,7 +42,6 @@ struct StaticEntry {
     int variant, emax, n, threads, tpb, nradices;
     int radices[kMaxPasses];
     hipError_t (*launch)(const LaunchShape&, const FftArgs&, hipStream_t);
-    int twiddle_lds_bytes;  // LDS the kernel adds after the data planes for its twiddle table (0: none)
 };
 extern const StaticEntry kStaticInstances[];
 extern const int kNumStaticInstances;

@@ -142,16 +142,11 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, Config* c
 }
 
 // Column mapping: lanes across adjacent transforms (columns) for coalesced strided access.
-// Preference: at least two workgroups per CU (LDS <= 80 KiB) with >= 4 columns per tile
-// (runs of >= 32 bytes; the XCD remap keeps the rest of each 128-byte line in one L2),
-// then more columns per tile, then the smaller register budget.
 bool cols_config(int64_t n, int64_t count, Config* c)
 {
     std::vector<int> radices = choose_radices(n);
     if (radices.empty()) return false;
     bool found = false;
-    Config best;
-    double best_score = -1e30;
     for (int tpb : {16, 8, 4, 2, 1}) {
         for (int emax : {16, 32}) {
             const int T = threads_for(n, radices, emax);
@@ -160,21 +155,18 @@ bool cols_config(int64_t n, int64_t count, Config* c)
             const bool need_lds = radices.size() > 1;
             const size_t lds = need_lds ? (size_t)(8 * plane) : 0;
             if (lds > kLdsLimit) continue;
-            const bool two_per_cu = lds <= kLdsLimit / 2 && tpb >= 4;
-            const double score = (two_per_cu ? 1000.0 : 0.0) + 10.0 * tpb - (emax == 32 ? 1.0 : 0.0);
-            if (found && score <= best_score) continue;
-            best_score = score;
+            c->radices = radices;
+            c->emax = emax;
+            c->threads = T;
+            c->tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, count));
+            c->tstride = 0;
+            c->plane = plane;
+            c->lds = lds;
             found = true;
-            best.radices = radices;
-            best.emax = emax;
-            best.threads = T;
-            best.tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, count));
-            best.tstride = 0;
-            best.plane = plane;
-            best.lds = lds;
+            break;
         }
+        if (found) break;
     }
-    if (found) *c = best;
     return found;
 }
 
@@ -199,15 +191,6 @@ int xcd_remap_for(bool cols_mapping, int tpb)
     const char* e = getenv("ASDFFT_XCD_REMAP");
     if (e != nullptr && (e[0] == '0' || e[0] == '1')) return e[0] - '0';
     return (cols_mapping && tpb < 16) ? 1 : 0;
-}
-
-// Tiles each persistent workgroup runs at least (so the next tile's loads overlap the
-// current tile).  ASDFFT_TILES_PER_BLOCK overrides (measurement).
-int tiles_per_block_for(int64_t tiles)
-{
-    const char* e = getenv("ASDFFT_TILES_PER_BLOCK");
-    if (e != nullptr && atoi(e) > 0) return atoi(e);
-    return tiles >= 512 ? 2 : 1;
 }
 
 // A Stockham launch.  Addressing: transform t at (t % cols)*inner + (t / cols)*outer.
@@ -260,29 +243,16 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     st->shape.lds_bytes = c.lds;
     st->shape.grid = (io.count + c.tpb - 1) / c.tpb;
     st->shape.static_id = -1;
-    st->shape.tiles_per_block = tiles_per_block_for(st->shape.grid);
     const char* ns = getenv("ASDFFT_NO_STATIC");
     if (ns == nullptr || ns[0] != '1') {
         st->shape.static_id = find_static(variant, c.emax, (int)n, c.threads, c.tpb, c.radices.data(),
                                           (int)c.radices.size());
-        if (st->shape.static_id >= 0) {
-            const int twb = kStaticInstances[st->shape.static_id].twiddle_lds_bytes;
-            if (twb > 0) {
-                // data planes (two of lds_plane floats) followed by the staged twiddle table
-                const size_t need = (size_t)8 * (size_t)c.plane + (size_t)twb;
-                if (need <= kLdsLimit) {
-                    st->shape.lds_bytes = need;
-                } else {
-                    st->shape.static_id = -1;
-                }
-            }
-        }
     }
     std::ostringstream os;
     os << "stockham variant=" << variant << " n=" << n << " count=" << io.count << " radices=";
     for (size_t i = 0; i < c.radices.size(); i++) os << (i ? "x" : "") << c.radices[i];
     os << " emax=" << c.emax << " T=" << c.threads << " tpb=" << c.tpb << " block=" << st->shape.block
-       << " lds=" << st->shape.lds_bytes << " grid=" << st->shape.grid;
+       << " lds=" << c.lds << " grid=" << st->shape.grid;
     if (st->shape.static_id >= 0) os << " static=" << st->shape.static_id;
     st->describe = os.str();
     return true;
@@ -526,11 +496,7 @@ bool real_rows(Kind kind, int64_t n, int64_t count, bool inverse, int src, int d
 
 }  // namespace
 
-int max_block_for(int emax)
-{
-    (void)emax;
-    return 512;
-}
+int max_block_for(int emax) { return emax <= 16 ? 256 : 512; }
 
 std::vector<int> choose_radices(int64_t n)
 {

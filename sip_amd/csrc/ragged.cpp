@@ -52,65 +52,6 @@ const char* kind_name(RaggedKind k)
     }
 }
 
-int64_t in_bytes_of(const RaggedEntry& e)
-{
-    switch (e.kind) {
-        case RaggedKind::C2C: return 8 * e.n;
-        case RaggedKind::R2C: return 4 * e.n;
-        default: return 8 * (e.n / 2 + 1);
-    }
-}
-
-int64_t out_bytes_of(const RaggedEntry& e)
-{
-    switch (e.kind) {
-        case RaggedKind::C2C: return 8 * e.n;
-        case RaggedKind::R2C: return 8 * (e.n / 2 + 1);
-        default: return 4 * e.n;
-    }
-}
-
-// Kernel variant, complex pass length, conjugation and load alignment of a class
-// (the same choices the batched planner makes for a plain 1D plan).
-struct ClassShape {
-    int variant;
-    int64_t n;  // complex length of the passes
-    int conj_in, conj_out;
-    bool aux;   // needs the half-length W_N^k table
-    int in_align, out_align;
-};
-
-bool shape_of(const RaggedEntry& e, ClassShape* s)
-{
-    const bool inv = e.inverse;
-    if (e.kind == RaggedKind::C2C) {
-        *s = {kVarRowsC2C, e.n, inv, inv, false, 8, 8};
-        return true;
-    }
-    const bool even = (e.n % 2) == 0 && e.n >= 4 && !choose_radices(e.n / 2).empty();
-    if (e.kind == RaggedKind::R2C) {
-        if (even) {
-            *s = {kVarR2CEven, e.n / 2, 0, inv, true, 8, 8};
-        } else {
-            *s = {kVarR2COdd, e.n, 0, inv, false, 4, 8};
-        }
-    } else {
-        if (even) {
-            *s = {kVarC2REven, e.n / 2, inv ? 0 : 1, 1, true, 8, 8};
-        } else {
-            *s = {kVarC2ROdd, e.n, inv, 0, false, 8, 4};
-        }
-    }
-    return true;
-}
-
-std::string class_key(const RaggedEntry& e)
-{
-    std::ostringstream os;
-    os << kind_name(e.kind) << ":" << (e.inverse ? "i" : "f") << ":" << e.n;
-    return os.str();
-}
-
 // Threads per transform so that every pass has EMAX / R butterflies per thread at most.
 int min_threads(int64_t n, const std::vector<int>& radices, int emax)
 {
@@ -150,6 +91,128 @@ void free_device(void* p)
 
 }  // namespace
 
+int64_t ragged_in_bytes(const RaggedEntry& e)
+{
+    switch (e.kind) {
+        case RaggedKind::C2C: return 8 * e.n;
+        case RaggedKind::R2C: return 4 * e.n;
+        default: return 8 * (e.n / 2 + 1);
+    }
+}
+
+int64_t ragged_out_bytes(const RaggedEntry& e)
+{
+    switch (e.kind) {
+        case RaggedKind::C2C: return 8 * e.n;
+        case RaggedKind::R2C: return 8 * (e.n / 2 + 1);
+        default: return 4 * e.n;
+    }
+}
+
+bool ragged_shape(const RaggedEntry& e, ClassShape* s)
+{
+    const bool inv = e.inverse;
+    if (e.kind == RaggedKind::C2C) {
+        *s = {kVarRowsC2C, e.n, inv, inv, false, 8, 8};
+        return true;
+    }
+    const bool even = (e.n % 2) == 0 && e.n >= 4 && !choose_radices(e.n / 2).empty();
+    if (e.kind == RaggedKind::R2C) {
+        if (even) {
+            *s = {kVarR2CEven, e.n / 2, 0, inv, true, 8, 8};
+        } else {
+            *s = {kVarR2COdd, e.n, 0, inv, false, 4, 8};
+        }
+    } else {
+        if (even) {
+            *s = {kVarC2REven, e.n / 2, inv ? 0 : 1, 1, true, 8, 8};
+        } else {
+            *s = {kVarC2ROdd, e.n, inv, 0, false, 8, 4};
+        }
+    }
+    return true;
+}
+
+std::string ragged_class_key(const RaggedEntry& e)
+{
+    std::ostringstream os;
+    os << kind_name(e.kind) << ":" << (e.inverse ? "i" : "f") << ":" << e.n;
+    return os.str();
+}
+
+std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, bool* unsupported)
+{
+    RaggedClassInfo& c = *cp;
+    ClassSchedule sch;
+    if (fixed != nullptr) {
+        sch = *fixed;
+    } else {
+        sch.radices = choose_radices(c.n);
+    }
+    int64_t prod = 1;
+    for (int r : sch.radices) prod *= r;
+    if (prod != c.n || sch.radices.empty() || (int)sch.radices.size() > kMaxPasses) {
+        return "schedule for " + c.key + " does not factor its length";
+    }
+    c.emax = 0;
+    for (int emax : {16, 32}) {
+        const int t = min_threads(c.n, sch.radices, emax);
+        if (t > 0 && t <= kRaggedMaxBlock) {
+            c.emax = emax;
+            sch.threads = std::max(sch.threads, t);
+            break;
+        }
+    }
+    if (c.emax == 0 || sch.threads > kRaggedMaxBlock) {
+        *unsupported = true;
+        return "no register budget holds " + c.key;
+    }
+    c.sched = sch;
+    return "";
+}
+
+size_t ragged_class_lds(const RaggedClassInfo& c, int tpb) { return class_lds(c, tpb); }
+
+bool ragged_device_class(const RaggedClassInfo& c, const ClassShape& s, int tpb, RaggedClass* rc,
+                         std::vector<TablePtr>* tables)
+{
+    *rc = RaggedClass{};
+    FftArgs& a = rc->a;
+    std::vector<int> offs;
+    TablePtr tw = upload_table(table_key(c.n, c.sched.radices), stockham_table((int)c.n, c.sched.radices, &offs));
+    if (!tw) return false;
+    tables->push_back(tw);
+    a.tw = tw->dev;
+    if (s.aux) {
+        TablePtr aux = upload_table("aux:" + std::to_string(c.nreal), roots_table(c.nreal, c.nreal / 2 + 1));
+        if (!aux) return false;
+        tables->push_back(aux);
+        a.tw_aux = aux->dev;
+    }
+    a.n = (int)c.n;
+    a.threads = c.sched.threads;
+    a.tpb = tpb;
+    a.lds_tstride = (int)(pad16(c.n) + 1);
+    a.lds_plane = tpb * a.lds_tstride;
+    a.conj_in = s.conj_in;
+    a.conj_out = s.conj_out;
+    a.count = c.transforms;
+    a.cols = 1;
+    a.in_stride = 1;
+    a.out_stride = 1;
+    a.passes = (int)c.sched.radices.size();
+    int span = 1;
+    for (size_t p = 0; p < c.sched.radices.size(); p++) {
+        a.pass[p].radix = c.sched.radices[p];
+        a.pass[p].span = span;
+        a.pass[p].magic = magic_for(span);
+        a.pass[p].tw_offset = offs[p];
+        span *= c.sched.radices[p];
+    }
+    rc->variant = c.variant;
+    return true;
+}
+
 RaggedPlan::~RaggedPlan()
 {
     int prev = -1;
@@ -188,7 +251,7 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
         if (e.n < 1 || e.count < 1) return where + "length and count must be positive";
         if (e.in_offset < 0 || e.out_offset < 0) return where + "offsets must be non-negative";
         ClassShape s;
-        shape_of(e, &s);
+        ragged_shape(e, &s);
         if (e.in_offset % s.in_align != 0 || e.out_offset % s.out_align != 0) {
             return where + "offsets must be multiples of the element size (" + std::to_string(s.in_align) + " / " +
                    std::to_string(s.out_align) + " bytes)";
@@ -198,12 +261,12 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
             return where + "length " + std::to_string(e.n) +
                    " is outside the ragged kernels (radix-{2..19}-smooth, at most 16384 complex points per pass)";
         }
-        const int64_t ib = in_bytes_of(e), ob = out_bytes_of(e);
+        const int64_t ib = ragged_in_bytes(e), ob = ragged_out_bytes(e);
         plan->in_extent = std::max<size_t>(plan->in_extent, (size_t)(e.in_offset + e.count * ib));
         plan->out_extent = std::max<size_t>(plan->out_extent, (size_t)(e.out_offset + e.count * ob));
         out_ranges.push_back({e.out_offset, e.out_offset + e.count * ob});
         inplace = inplace && e.kind == RaggedKind::C2C && e.in_offset == e.out_offset;
-        const std::string key = class_key(e);
+        const std::string key = ragged_class_key(e);
         auto it = class_of.find(key);
         if (it == class_of.end()) {
             class_of[key] = (int)plan->classes.size();
@@ -236,31 +299,8 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
     for (size_t ci = 0; ci < plan->classes.size(); ci++) {
         RaggedClassInfo& c = plan->classes[ci];
         auto f = opt.fixed.find(c.key);
-        ClassSchedule sch;
-        if (f != opt.fixed.end()) {
-            sch = f->second;
-        } else {
-            sch.radices = choose_radices(c.n);
-        }
-        int64_t prod = 1;
-        for (int r : sch.radices) prod *= r;
-        if (prod != c.n || sch.radices.empty() || (int)sch.radices.size() > kMaxPasses) {
-            return "schedule for " + c.key + " does not factor its length";
-        }
-        c.emax = 0;
-        for (int emax : {16, 32}) {
-            const int t = min_threads(c.n, sch.radices, emax);
-            if (t > 0 && t <= kRaggedMaxBlock) {
-                c.emax = emax;
-                sch.threads = std::max(sch.threads, t);
-                break;
-            }
-        }
-        if (c.emax == 0 || sch.threads > kRaggedMaxBlock) {
-            *unsupported = true;
-            return "no register budget holds " + c.key;
-        }
-        c.sched = sch;
+        const std::string why = ragged_schedule(&c, f != opt.fixed.end() ? &f->second : nullptr, unsupported);
+        if (!why.empty()) return why;
     }
 
     // ---- launches: one per register budget
@@ -308,7 +348,7 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
             const int64_t first = (int64_t)plan->offsets.size() / 2;
             for (int ei : class_entries[ci]) {
                 const RaggedEntry& e = entries[ei];
-                const int64_t ib = in_bytes_of(e), ob = out_bytes_of(e);
+                const int64_t ib = ragged_in_bytes(e), ob = ragged_out_bytes(e);
                 for (int64_t t = 0; t < e.count; t++) {
                     plan->offsets.push_back(e.in_offset + t * ib);
                     plan->offsets.push_back(e.out_offset + t * ob);
@@ -352,39 +392,7 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
             const RaggedClassInfo& c = plan->classes[ci];
             const ClassShape& s = shapes[ci];
             RaggedClass rc{};
-            FftArgs& a = rc.a;
-            std::vector<int> offs;
-            TablePtr tw = upload_table(table_key(c.n, c.sched.radices), stockham_table((int)c.n, c.sched.radices, &offs));
-            if (!tw) return "";
-            plan->tables.push_back(tw);
-            a.tw = tw->dev;
-            if (s.aux) {
-                TablePtr aux = upload_table("aux:" + std::to_string(c.nreal), roots_table(c.nreal, c.nreal / 2 + 1));
-                if (!aux) return "";
-                plan->tables.push_back(aux);
-                a.tw_aux = aux->dev;
-            }
-            a.n = (int)c.n;
-            a.threads = c.sched.threads;
-            a.tpb = c.sched.tpb;
-            a.lds_tstride = (int)(pad16(c.n) + 1);
-            a.lds_plane = c.sched.tpb * a.lds_tstride;
-            a.conj_in = s.conj_in;
-            a.conj_out = s.conj_out;
-            a.count = c.transforms;
-            a.cols = 1;
-            a.in_stride = 1;
-            a.out_stride = 1;
-            a.passes = (int)c.sched.radices.size();
-            int span = 1;
-            for (size_t p = 0; p < c.sched.radices.size(); p++) {
-                a.pass[p].radix = c.sched.radices[p];
-                a.pass[p].span = span;
-                a.pass[p].magic = magic_for(span);
-                a.pass[p].tw_offset = offs[p];
-                span *= c.sched.radices[p];
-            }
-            rc.variant = c.variant;
+            if (!ragged_device_class(c, s, c.sched.tpb, &rc, &plan->tables)) return "";
             L.dev_classes.push_back(rc);
         }
         if (hipMalloc(reinterpret_cast<void**>(&L.d_classes), L.dev_classes.size() * sizeof(RaggedClass)) !=
@@ -584,7 +592,7 @@ ClassSchedule tune_class(const RaggedClassInfo& c, const RaggedOptions& base, Sc
     e.kind = c.kind;
     e.inverse = c.inverse;
     e.n = c.nreal;
-    const int64_t ib = in_bytes_of(e), ob = out_bytes_of(e);
+    const int64_t ib = ragged_in_bytes(e), ob = ragged_out_bytes(e);
     const int64_t want = (int64_t)(kTuneBytes / (double)(ib + ob)) + 1;
     e.count = std::max<int64_t>(c.transforms, std::min<int64_t>(want, 1 << 16));
     const size_t need = (size_t)std::max(ib, ob) * (size_t)e.count;

@@ -110,6 +110,27 @@ struct RaggedPlan {
     ~RaggedPlan();
 };
 
+// ---- class helpers (shared with the serving loop) ------------------------------------
+// Kernel variant, complex pass length, conjugation and load alignment of a class (the
+// same choices the batched planner makes for a plain 1D plan).
+struct ClassShape {
+    int variant;
+    int64_t n;  // complex length of the passes
+    int conj_in, conj_out;
+    bool aux;   // needs the half-length W_N^k table
+    int in_align, out_align;
+};
+bool ragged_shape(const RaggedEntry& e, ClassShape* s);
+std::string ragged_class_key(const RaggedEntry& e);  // "c2c:f:4096", ...
+int64_t ragged_in_bytes(const RaggedEntry& e);       // bytes of one transform's input
+int64_t ragged_out_bytes(const RaggedEntry& e);
+// Fills c->sched (the given schedule, or the planner's default) and c->emax; "" on success.
+std::string ragged_schedule(RaggedClassInfo* c, const ClassSchedule* fixed, bool* unsupported);
+size_t ragged_class_lds(const RaggedClassInfo& c, int tpb);
+// Device descriptor of a class at tpb transforms per tile; uploads its twiddle tables.
+bool ragged_device_class(const RaggedClassInfo& c, const ClassShape& s, int tpb, RaggedClass* rc,
+                         std::vector<TablePtr>* tables);
+
 // Validates the entries and builds the plan ("" on success, otherwise the reason).
 // `unsupported` is set when the entries are valid but outside what the kernels handle.
 std::string build_ragged(const std::vector<RaggedEntry>& entries, const RaggedOptions& opt, RaggedPlan* plan,

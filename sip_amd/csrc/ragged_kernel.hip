@@ -28,8 +28,8 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& r, const FftArgs& 
                 __builtin_amdgcn_readfirstlane(a.tpb), __builtin_amdgcn_readfirstlane(a.lds_tstride)};
     float* lre = smem;
     float* lim = smem + __builtin_amdgcn_readfirstlane(a.lds_plane);
-    int tt, tid;
-    lane_map<kMapRows>(g, tt, tid);
+    int tt = (int)threadIdx.x / g.T;  // rows mapping: lanes along a transform
+    int tid = (int)threadIdx.x - tt * g.T;
     const bool lane_ok = tt < g.tpb;
     const bool live = lane_ok && tt < count;
     if (!lane_ok) {
@@ -143,7 +143,7 @@ hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu)
 {
     const void* fn = ragged_kernel_fn(emax);
     if (lds_bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
         if (e != hipSuccess) return e;
     }
     int n = 0;

@@ -1,0 +1,666 @@
+// Serving loop (see server.h).
+#include "server.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "planner.h"
+#include "ragged.h"
+
+namespace asdfft {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int kPoolClasses = 4096;           // resident ragged class descriptors
+constexpr int64_t kSlotTransforms = 1 << 16;  // ragged transforms per batch
+constexpr int kSlotTiles = 1 << 16;
+constexpr int kQueueStrideInts = 32;
+constexpr size_t kServeLdsCap = 80 * 1024;   // per-class tile LDS (two workgroups per CU)
+constexpr int kBlock = 512;
+constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12;
+
+struct Req {
+    ServeRequest r;
+    bool ragged = false;
+    RaggedEntry entry;      // ragged: one entry of `batch` transforms
+    std::string plain_key;  // plain: plan cache key
+    size_t bytes = 0;
+    Clock::time_point submitted, completed;
+    int status = -1;        // -1 in flight, 0 done, 3 device error
+};
+
+struct PoolClass {
+    RaggedClassInfo info;
+    ClassShape shape{};
+    int tpb = 1;
+    size_t lds = 0;
+};
+
+struct PlainPlan {
+    BuiltPlan built;
+    std::vector<void*> workspace;  // one per slot (execs on different slots overlap)
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    RaggedTile* d_tiles = nullptr;
+    int64_t* d_offsets = nullptr;
+    int* d_queue = nullptr;
+    RaggedTile* h_tiles = nullptr;  // pinned staging
+    int64_t* h_offsets = nullptr;
+    std::vector<uint64_t> tickets;
+    size_t bytes = 0;
+    bool busy = false;
+    bool launched = false;
+    int error = 0;
+    std::thread waiter;
+};
+
+int64_t kind_in_bytes(int type, int64_t inner_points, int64_t last)
+{
+    // inner_points: points of the leading FFT axes; last: the contiguous axis length
+    if (type == kTypeC2C) return 8 * inner_points * last;
+    if (type == kTypeR2C) return 4 * inner_points * last;
+    return 8 * inner_points * (last / 2 + 1);
+}
+
+int64_t kind_out_bytes(int type, int64_t inner_points, int64_t last)
+{
+    if (type == kTypeC2C) return 8 * inner_points * last;
+    if (type == kTypeR2C) return 8 * inner_points * (last / 2 + 1);
+    return 4 * inner_points * last;
+}
+
+Kind plan_kind(int type)
+{
+    if (type == kTypeR2C) return Kind::R2C;
+    if (type == kTypeC2R) return Kind::C2R;
+    return Kind::C2C;
+}
+
+RaggedKind ragged_kind(int type)
+{
+    if (type == kTypeR2C) return RaggedKind::R2C;
+    if (type == kTypeC2R) return RaggedKind::C2R;
+    return RaggedKind::C2C;
+}
+
+}  // namespace
+
+class Server {
+public:
+    ServeConfig cfg;
+    int device = 0;
+    int cus = 256;
+    std::mutex mu;
+    std::condition_variable cv;  // scheduler, waiters and serve_wait callers
+    std::deque<uint64_t> pending;
+    std::unordered_map<uint64_t, Req> reqs;
+    uint64_t next_ticket = 1;
+    size_t inflight = 0;
+    size_t resident = 0;
+    bool stop = false;
+    ServeStats stats;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::thread scheduler;
+    hipStream_t setup = nullptr;  // uploads of new resident state
+    // resident ragged classes (touched by the scheduler thread only, after creation)
+    RaggedClass* d_pool = nullptr;
+    std::vector<PoolClass> pool;
+    std::map<std::string, int> pool_index;
+    std::vector<TablePtr> tables;
+    std::mutex plan_mu;  // plain plans are created by submitters and read by the scheduler
+    std::map<std::string, std::unique_ptr<PlainPlan>> plans;
+    std::map<std::pair<int, size_t>, int> occupancy;  // (emax, lds) -> workgroups per CU
+
+    ~Server();
+    bool init(std::string* why);
+    void scheduler_loop();
+    void waiter_loop(Slot* s);
+    int launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch);
+    int pool_class(const RaggedEntry& e);
+    PlainPlan* plain_plan(const Req& r, std::string* why);
+    int per_cu(int emax, size_t lds);
+};
+
+bool Server::init(std::string* why)
+{
+    if (!device_available() || hipGetDevice(&device) != hipSuccess) {
+        *why = "no HIP device";
+        return false;
+    }
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n > 0) cus = n;
+    if (hipStreamCreateWithFlags(&setup, hipStreamNonBlocking) != hipSuccess) {
+        *why = "stream creation failed";
+        return false;
+    }
+    if (hipMalloc(reinterpret_cast<void**>(&d_pool), sizeof(RaggedClass) * kPoolClasses) != hipSuccess) {
+        *why = "device allocation failed";
+        return false;
+    }
+    resident += sizeof(RaggedClass) * kPoolClasses;
+    for (int emax : {16, 32}) {
+        int pc = 0;
+        if (ragged_prepare(emax, 160 * 1024, kBlock, &pc) != hipSuccess) {
+            *why = "ragged kernel setup failed";
+            return false;
+        }
+    }
+    for (int i = 0; i < cfg.slots; i++) {
+        auto s = std::make_unique<Slot>();
+        // non-blocking streams: uploads of new classes on other streams never wait for batches
+        bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&s->done, hipEventDisableTiming) == hipSuccess &&
+                  hipMalloc(reinterpret_cast<void**>(&s->d_tiles), sizeof(RaggedTile) * kSlotTiles) == hipSuccess &&
+                  hipMalloc(reinterpret_cast<void**>(&s->d_offsets), sizeof(int64_t) * 2 * kSlotTransforms) ==
+                      hipSuccess &&
+                  hipMalloc(reinterpret_cast<void**>(&s->d_queue), sizeof(int) * 2 * kQueueStrideInts) == hipSuccess &&
+                  hipMemset(s->d_queue, 0, sizeof(int) * 2 * kQueueStrideInts) == hipSuccess &&
+                  hipHostMalloc(reinterpret_cast<void**>(&s->h_tiles), sizeof(RaggedTile) * kSlotTiles, 0) ==
+                      hipSuccess &&
+                  hipHostMalloc(reinterpret_cast<void**>(&s->h_offsets), sizeof(int64_t) * 2 * kSlotTransforms, 0) ==
+                      hipSuccess;
+        resident += sizeof(RaggedTile) * kSlotTiles + sizeof(int64_t) * 2 * kSlotTransforms;
+        slots.push_back(std::move(s));
+        if (!ok) {
+            *why = "slot allocation failed";
+            return false;
+        }
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        *why = "device error during setup";
+        return false;
+    }
+    if (resident >= cfg.hbm_budget) {
+        *why = "HBM budget smaller than the server's resident state";
+        return false;
+    }
+    for (auto& s : slots) {
+        Slot* sp = s.get();
+        sp->waiter = std::thread([this, sp] { waiter_loop(sp); });
+    }
+    scheduler = std::thread([this] { scheduler_loop(); });
+    return true;
+}
+
+Server::~Server()
+{
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        stop = true;
+    }
+    cv.notify_all();
+    if (scheduler.joinable()) scheduler.join();
+    for (auto& s : slots) {
+        if (s->waiter.joinable()) s->waiter.join();
+    }
+    int prev = -1;
+    const bool have = hipGetDevice(&prev) == hipSuccess;
+    if (have && prev != device) (void)hipSetDevice(device);
+    (void)hipDeviceSynchronize();
+    for (auto& s : slots) {
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->done) (void)hipEventDestroy(s->done);
+        if (s->d_tiles) (void)hipFree(s->d_tiles);
+        if (s->d_offsets) (void)hipFree(s->d_offsets);
+        if (s->d_queue) (void)hipFree(s->d_queue);
+        if (s->h_tiles) (void)hipHostFree(s->h_tiles);
+        if (s->h_offsets) (void)hipHostFree(s->h_offsets);
+    }
+    for (auto& kv : plans) {
+        for (void* w : kv.second->workspace) {
+            if (w) (void)hipFree(w);
+        }
+    }
+    if (d_pool) (void)hipFree(d_pool);
+    if (setup) (void)hipStreamDestroy(setup);
+    if (have && prev != device) (void)hipSetDevice(prev);
+}
+
+int Server::per_cu(int emax, size_t lds)
+{
+    auto it = occupancy.find({emax, lds});
+    if (it != occupancy.end()) return it->second;
+    int pc = 0;
+    if (ragged_prepare(emax, lds, kBlock, &pc) != hipSuccess || pc < 1) pc = 1;
+    occupancy[{emax, lds}] = pc;
+    return pc;
+}
+
+// Index of the resident class for this entry (created and uploaded on first use), or -1.
+int Server::pool_class(const RaggedEntry& e)
+{
+    const std::string key = ragged_class_key(e);
+    auto it = pool_index.find(key);
+    if (it != pool_index.end()) return it->second;
+    if ((int)pool.size() >= kPoolClasses) return -1;
+    PoolClass pc;
+    ragged_shape(e, &pc.shape);
+    pc.info.key = key;
+    pc.info.kind = e.kind;
+    pc.info.inverse = e.inverse;
+    pc.info.variant = pc.shape.variant;
+    pc.info.n = pc.shape.n;
+    pc.info.nreal = e.n;
+    pc.info.transforms = 1;
+    bool unsup = false;
+    if (!ragged_schedule(&pc.info, nullptr, &unsup).empty()) return -1;
+    const size_t one = ragged_class_lds(pc.info, 1);
+    const size_t cap = std::max(kServeLdsCap, one);
+    int tpb = std::max(1, kBlock / pc.info.sched.threads);
+    while (tpb > 1 && ragged_class_lds(pc.info, tpb) > cap) tpb--;
+    pc.tpb = tpb;
+    pc.lds = ragged_class_lds(pc.info, tpb);
+    RaggedClass rc;
+    if (!ragged_device_class(pc.info, pc.shape, tpb, &rc, &tables)) return -1;
+    const int idx = (int)pool.size();
+    if (hipMemcpyAsync(d_pool + idx, &rc, sizeof(rc), hipMemcpyHostToDevice, setup) != hipSuccess ||
+        hipStreamSynchronize(setup) != hipSuccess) {
+        return -1;
+    }
+    pool.push_back(pc);
+    pool_index[key] = idx;
+    return idx;
+}
+
+PlainPlan* Server::plain_plan(const Req& q, std::string* why)
+{
+    std::lock_guard<std::mutex> lock(plan_mu);
+    auto it = plans.find(q.plain_key);
+    if (it != plans.end()) return it->second.get();
+    PlanSpec spec;
+    spec.kind = plan_kind(q.r.type);
+    spec.inverse = q.r.inverse;
+    spec.dims.assign(q.r.dims, q.r.dims + q.r.rank);
+    spec.batch = q.r.batch;
+    auto p = std::make_unique<PlainPlan>();
+    *why = build_plan(spec, &p->built);
+    if (!why->empty()) return nullptr;
+    for (const Step& st : p->built.steps) {
+        if (!st.tables_ready) {
+            *why = "plan tables could not be uploaded";
+            return nullptr;
+        }
+    }
+    p->workspace.assign(slots.size(), nullptr);
+    if (p->built.workspace > 0) {
+        for (auto& w : p->workspace) {
+            if (hipMalloc(&w, p->built.workspace) != hipSuccess) {
+                *why = "workspace allocation failed";
+                return nullptr;
+            }
+        }
+        std::lock_guard<std::mutex> lock(mu);
+        resident += p->built.workspace * slots.size();
+    }
+    PlainPlan* raw = p.get();
+    plans[q.plain_key] = std::move(p);
+    return raw;
+}
+
+int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch)
+{
+    // ---- ragged part: transforms grouped per resident class
+    std::map<int, std::vector<int64_t>> per_class;  // pool index -> (in, out) address pairs
+    std::vector<Req*> plain;
+    for (Req* q : batch) {
+        int ci = q->ragged ? pool_class(q->entry) : -1;
+        if (ci < 0) {
+            plain.push_back(q);
+            continue;
+        }
+        const int64_t ib = ragged_in_bytes(q->entry), ob = ragged_out_bytes(q->entry);
+        auto& v = per_class[ci];
+        const int64_t in0 = (int64_t)reinterpret_cast<uintptr_t>(q->r.in);
+        const int64_t out0 = (int64_t)reinterpret_cast<uintptr_t>(q->r.out);
+        for (int64_t t = 0; t < q->entry.count; t++) {
+            v.push_back(in0 + t * ib);
+            v.push_back(out0 + t * ob);
+        }
+    }
+    int64_t nt = 0;
+    struct Costed {
+        double cost;
+        RaggedTile t;
+        int emax;
+    };
+    std::vector<Costed> tiles;
+    size_t lds[2] = {0, 0};
+    for (auto& kv : per_class) {
+        const PoolClass& pc = pool[kv.first];
+        const int64_t count = (int64_t)kv.second.size() / 2;
+        memcpy(s.h_offsets + 2 * nt, kv.second.data(), kv.second.size() * sizeof(int64_t));
+        const double per = (double)pc.info.n * (double)(pc.info.sched.radices.size() + 1);
+        for (int64_t f = 0; f < count; f += pc.tpb) {
+            RaggedTile t;
+            t.cls = kv.first;
+            t.count = (int)std::min<int64_t>(pc.tpb, count - f);
+            t.first = nt + f;
+            tiles.push_back({per * t.count, t, pc.info.emax});
+        }
+        lds[pc.info.emax == 16 ? 0 : 1] = std::max(lds[pc.info.emax == 16 ? 0 : 1], pc.lds);
+        nt += count;
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [](const Costed& a, const Costed& b) {
+        if (a.emax != b.emax) return a.emax < b.emax;
+        return a.cost > b.cost;
+    });
+    int64_t n16 = 0;
+    for (size_t i = 0; i < tiles.size(); i++) {
+        s.h_tiles[i] = tiles[i].t;
+        if (tiles[i].emax == 16) n16++;
+    }
+    const int64_t n32 = (int64_t)tiles.size() - n16;
+    if (!tiles.empty()) {
+        if (hipMemcpyAsync(s.d_offsets, s.h_offsets, sizeof(int64_t) * 2 * nt, hipMemcpyHostToDevice, s.stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(s.d_tiles, s.h_tiles, sizeof(RaggedTile) * tiles.size(), hipMemcpyHostToDevice,
+                           s.stream) != hipSuccess) {
+            return 3;
+        }
+        for (int g = 0; g < 2; g++) {
+            const int64_t count = g == 0 ? n16 : n32;
+            if (count == 0) continue;
+            const int emax = g == 0 ? 16 : 32;
+            RaggedArgs a{};
+            a.in_base = nullptr;  // offsets hold absolute device addresses
+            a.out_base = nullptr;
+            a.offsets = s.d_offsets;
+            a.queue = s.d_queue + g * kQueueStrideInts;
+            a.ntiles = (int)count;
+            const int64_t grid = std::min<int64_t>(count, (int64_t)per_cu(emax, lds[g]) * cus);
+            if (launch_ragged(emax, grid, kBlock, lds[g], a, d_pool, s.d_tiles + (g == 0 ? 0 : n16), s.stream) !=
+                hipSuccess) {
+                return 3;
+            }
+        }
+    }
+    // ---- plain part: cached plans, launched on this slot's stream
+    for (Req* q : plain) {
+        std::string why;
+        PlainPlan* p = plain_plan(*q, &why);
+        if (p == nullptr) return 3;
+        char* bufs[3] = {static_cast<char*>(const_cast<void*>(q->r.in)), static_cast<char*>(q->r.out),
+                         static_cast<char*>(p->workspace[slot_index])};
+        for (const Step& st : p->built.steps) {
+            hipError_t e;
+            if (st.direct) {
+                DftArgs a = st.dargs;
+                a.in = bufs[st.src] + st.src_offset;
+                a.out = bufs[st.dst] + st.dst_offset;
+                e = launch_dft_direct(st.shape, a, s.stream);
+            } else {
+                FftArgs a = st.args;
+                a.in = bufs[st.src] + st.src_offset;
+                a.out = bufs[st.dst] + st.dst_offset;
+                e = launch_stockham(st.shape, a, s.stream);
+            }
+            if (e != hipSuccess) return 3;
+        }
+    }
+    return hipEventRecord(s.done, s.stream) == hipSuccess ? 0 : 3;
+}
+
+void Server::scheduler_loop()
+{
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        int si = -1;
+        cv.wait(lk, [&] {
+            if (stop) return true;
+            if (pending.empty()) return false;
+            for (size_t i = 0; i < slots.size(); i++) {
+                if (!slots[i]->busy) {
+                    si = (int)i;
+                    return true;
+                }
+            }
+            return false;
+        });
+        if (stop) break;
+        // admission: arrival order while the budget and the slot's tables allow
+        std::vector<Req*> batch;
+        std::vector<uint64_t> tickets;
+        size_t add = 0;
+        int64_t transforms = 0;
+        const size_t budget = cfg.hbm_budget - std::min(cfg.hbm_budget, resident);
+        while (!pending.empty() && (int)batch.size() < cfg.max_batch) {
+            Req& q = reqs[pending.front()];
+            if (inflight + add + q.bytes > budget) break;
+            if (!cfg.batching && !batch.empty()) break;
+            if (q.ragged && transforms + q.entry.count > kSlotTransforms) break;
+            if (q.ragged) transforms += q.entry.count;
+            batch.push_back(&q);
+            tickets.push_back(pending.front());
+            add += q.bytes;
+            pending.pop_front();
+        }
+        if (batch.empty()) {
+            // the head request waits for in-flight batches to release budget
+            cv.wait(lk);
+            continue;
+        }
+        Slot& s = *slots[si];
+        s.busy = true;
+        s.tickets = tickets;
+        s.bytes = add;
+        inflight += add;
+        stats.peak_inflight_bytes = std::max<uint64_t>(stats.peak_inflight_bytes, inflight);
+        stats.batches++;
+        lk.unlock();
+        const int st = launch_batch(s, si, batch);
+        lk.lock();
+        s.error = st;
+        s.launched = true;
+        cv.notify_all();
+    }
+}
+
+void Server::waiter_loop(Slot* s)
+{
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cv.wait(lk, [&] { return stop || s->launched; });
+        if (!s->launched) break;  // stopping with nothing in flight
+        lk.unlock();
+        int st = s->error;
+        if (hipEventSynchronize(s->done) != hipSuccess) st = 3;
+        const Clock::time_point now = Clock::now();
+        lk.lock();
+        for (uint64_t t : s->tickets) {
+            auto it = reqs.find(t);
+            if (it == reqs.end()) continue;
+            it->second.completed = now;
+            it->second.status = st;
+            stats.completed++;
+        }
+        inflight -= s->bytes;
+        s->bytes = 0;
+        s->tickets.clear();
+        s->launched = false;
+        s->busy = false;
+        cv.notify_all();
+    }
+}
+
+Server* serve_create(const ServeConfig& cfg, std::string* why)
+{
+    if (cfg.slots < 1 || cfg.slots > 8 || cfg.max_batch < 1 || cfg.hbm_budget == 0) {
+        *why = "invalid server configuration";
+        return nullptr;
+    }
+    auto* s = new Server();
+    s->cfg = cfg;
+    if (!s->init(why)) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void serve_destroy(Server* s)
+{
+    if (s == nullptr) return;
+    // let in-flight batches finish before the threads stop
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv.wait(lk, [&] {
+            if (!s->pending.empty()) return false;
+            for (auto& sl : s->slots) {
+                if (sl->busy) return false;
+            }
+            return true;
+        });
+    }
+    delete s;
+}
+
+int serve_submit(Server* s, const ServeRequest& r, uint64_t* ticket, std::string* why)
+{
+    if (r.type != kTypeC2C && r.type != kTypeC2R && r.type != kTypeR2C) {
+        *why = "unsupported type";
+        return 1;
+    }
+    if (r.rank < 1 || r.rank > 2 || r.batch < 1 || r.in == nullptr || r.out == nullptr) {
+        *why = "invalid rank, batch or buffer";
+        return 1;
+    }
+    int64_t inner = 1;
+    for (int d = 0; d < r.rank; d++) {
+        if (r.dims[d] < 1 || r.dims[d] > ((int64_t)1 << 27)) {
+            *why = "invalid size";
+            return 1;
+        }
+        if (d + 1 < r.rank) inner *= r.dims[d];
+    }
+    const int64_t last = r.dims[r.rank - 1];
+    Req q;
+    q.r = r;
+    q.bytes = (size_t)((kind_in_bytes(r.type, inner, last) + kind_out_bytes(r.type, inner, last)) * r.batch);
+    q.submitted = Clock::now();
+    if (r.rank == 1) {
+        q.entry.kind = ragged_kind(r.type);
+        q.entry.inverse = r.inverse;
+        q.entry.n = last;
+        q.entry.count = r.batch;
+        ClassShape cs;
+        ragged_shape(q.entry, &cs);
+        q.ragged = s->cfg.batching && cs.n >= 2 && cs.n <= 16384 && !choose_radices(cs.n).empty() &&
+                   r.batch <= kSlotTransforms && reinterpret_cast<uintptr_t>(r.in) % cs.in_align == 0 &&
+                   reinterpret_cast<uintptr_t>(r.out) % cs.out_align == 0;
+    }
+    if (!q.ragged) {
+        // the plan is built (and its tables made resident) at submission, so a plan
+        // failure is reported here and never inside a batch
+        char key[160];
+        snprintf(key, sizeof(key), "%d:%d:%d:%lld:%lld:%lld", r.type, (int)r.inverse, r.rank, (long long)r.dims[0],
+                 (long long)(r.rank > 1 ? r.dims[1] : 0), (long long)r.batch);
+        q.plain_key = key;
+        std::string w;
+        if (s->plain_plan(q, &w) == nullptr) {
+            *why = w;
+            return 2;
+        }
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
+    const size_t budget = s->cfg.hbm_budget - std::min(s->cfg.hbm_budget, s->resident);
+    if (q.bytes > budget) {
+        *why = "request larger than the HBM budget";
+        return 1;
+    }
+    const uint64_t t = s->next_ticket++;
+    s->reqs[t] = q;
+    s->pending.push_back(t);
+    s->stats.submitted++;
+    if (q.ragged) {
+        s->stats.ragged_requests++;
+    } else {
+        s->stats.plain_requests++;
+    }
+    *ticket = t;
+    s->cv.notify_all();
+    return 0;
+}
+
+int serve_wait(Server* s, uint64_t ticket, int64_t timeout_us, double* latency_us)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->reqs.find(ticket) == s->reqs.end()) return 1;
+    auto ready = [&] {
+        auto f = s->reqs.find(ticket);
+        return f == s->reqs.end() || f->second.status >= 0;
+    };
+    if (timeout_us < 0) {
+        s->cv.wait(lk, ready);
+    } else if (!s->cv.wait_for(lk, std::chrono::microseconds(timeout_us), ready)) {
+        return 4;
+    }
+    auto f = s->reqs.find(ticket);
+    if (f == s->reqs.end()) return 1;  // retired by another waiter
+    Req& q = f->second;
+    const int st = q.status;
+    if (latency_us != nullptr) {
+        *latency_us = std::chrono::duration<double, std::micro>(q.completed - q.submitted).count();
+    }
+    s->reqs.erase(ticket);
+    return st;
+}
+
+int serve_replay(Server* s, const ServeRequest* reqs, const double* arrival_us, int64_t n, double* latency_us)
+{
+    std::vector<uint64_t> tickets((size_t)n, 0);
+    std::vector<Clock::time_point> due((size_t)n);
+    const Clock::time_point t0 = Clock::now() + std::chrono::milliseconds(1);
+    int worst = 0;
+    for (int64_t i = 0; i < n; i++) {
+        due[i] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double, std::micro>(
+                          arrival_us[i]));
+        while (Clock::now() < due[i]) {
+        }
+        std::string why;
+        if (serve_submit(s, reqs[i], &tickets[i], &why) != 0) {
+            tickets[i] = 0;
+            worst = 1;
+        }
+    }
+    for (int64_t i = 0; i < n; i++) {
+        if (tickets[i] == 0) {
+            latency_us[i] = -1.0;
+            continue;
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv.wait(lk, [&] { return s->reqs[tickets[i]].status >= 0; });
+        Req& q = s->reqs[tickets[i]];
+        latency_us[i] = std::chrono::duration<double, std::micro>(q.completed - due[i]).count();
+        if (q.status != 0) worst = std::max(worst, q.status);
+        s->reqs.erase(tickets[i]);
+    }
+    return worst;
+}
+
+void serve_stats(Server* s, ServeStats* out)
+{
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = s->stats;
+    out->resident_bytes = s->resident;
+}
+
+}  // namespace asdfft

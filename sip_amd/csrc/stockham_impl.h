@@ -270,17 +270,6 @@ __device__ __forceinline__ float2 ldg2(const float* p, int64_t idx)
     return reinterpret_cast<const float2*>(p)[idx];
 }
 
-// Workgroup barrier for LDS exchanges only.  __syncthreads() also waits for every
-// outstanding global load (vmcnt 0), which would stall the prefetch of the next tile at
-// the first exchange; here only LDS traffic (lgkmcnt) is drained.  The empty asm
-// statements keep the compiler from moving LDS accesses across the barrier.
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
 // x.y ^= sign bit when mask is 0x80000000 (conjugation without a branch)
 __device__ __forceinline__ float2 conj_if(float2 x, uint32_t mask)
 {
@@ -367,38 +356,10 @@ __device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int
 // RT: runtime geometry (the pass loop dispatches on the radix at run time).  The
 // opaque copies stop the compiler from hoisting every radix case's address
 // arithmetic out of that loop and keeping all of it live at once (hundreds of VGPRs).
-// First-pass gather from HBM: butterfly b of this thread reads elements j + r*n/R,
-// j = tid + b*T (lane-contiguous float2 streams), then applies conj_in.
-template <int EMAX, int R, int MAP, int PRO>
-__device__ __forceinline__ void load_first(const FftArgs& a, const Io& io, int n, int T, int tid, bool live,
-                                           float2 (&v)[EMAX])
-{
-    constexpr int B = EMAX / R;
-    const int nr = n / R;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = tid + b * T;
-        if (live && j < nr) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
-        }
-    }
-    if constexpr (PRO == kProComplex) {
-        if (io.cin) {  // uniform: one scalar branch instead of a XOR per point
-#pragma unroll
-            for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
-        }
-    }
-}
-
-// PRE: the first pass's inputs were already gathered (prefetched) by the caller into pre.
-// TWL: the twiddle table was staged in LDS at twl (reads count on lgkmcnt, so they do not
-// drain outstanding prefetch loads the way global twiddle loads on vmcnt would).
-template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool PRE = false, bool TWL = false>
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT>
 __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
                                               bool first, bool last, float* lre, float* lim, int tt, int tid,
-                                              bool live, const Io& io, const float2 (&pre)[EMAX],
-                                              const float2* twl = nullptr)
+                                              bool live, const Io& io)
 {
     float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
     constexpr int B = EMAX / R;
@@ -406,11 +367,8 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose);
     if constexpr (RT) {
         asm volatile("" : "+s"(g.n), "+s"(g.T));
+        asm volatile("" : "+v"(tid));
     }
-    // Keeps each pass's address arithmetic and twiddle loads inside the pass (the
-    // compiler would otherwise hoist later passes' loads above the barriers and hold
-    // them in registers through the earlier passes).
-    asm volatile("" : "+v"(tid));
     const int n = g.n;
     const int nr = n / R;
     const int T = g.T;
@@ -428,11 +386,19 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 
     // 1. gather
     if (first) {
-        if constexpr (PRE) {
 #pragma unroll
-            for (int e = 0; e < EMAX; ++e) v[e] = pre[e];
-        } else {
-            load_first<EMAX, R, MAP, PRO>(a, io, n, T, tid, live, v);
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (live && j < nr) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
+            }
+        }
+        if constexpr (PRO == kProComplex) {
+            if (io.cin) {  // uniform: one scalar branch instead of a XOR per point
+#pragma unroll
+                for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
+            }
         }
     } else {
 #pragma unroll
@@ -450,12 +416,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     }
 
     // 2. twiddle + butterfly
-    const float2* twp;
-    if constexpr (TWL) {
-        twp = twl + tw_offset;
-    } else {
-        twp = reinterpret_cast<const float2*>(a.tw) + tw_offset;
-    }
+    const float2* twp = reinterpret_cast<const float2*>(a.tw) + tw_offset;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = tid + b * T;
@@ -472,7 +433,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 
     // 3. scatter
     if (!last || kEpiViaLds) {
-        lds_barrier();
+        __syncthreads();
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
@@ -489,7 +450,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 }
             }
         }
-        lds_barrier();
+        __syncthreads();
     } else {
         if constexpr (EPI == kEpiComplex || EPI == kEpiHalf) {
             if (io.cout) {  // uniform branch
@@ -516,9 +477,8 @@ __device__ __forceinline__ void maybe_pass(const FftArgs& a, const Geo& g, int p
         const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
         const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
         const int tw_offset = __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
-        float2 unused[EMAX];
         stockham_pass<EMAX, R, MAP, PRO, EPI, true>(a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre,
-                                                    lim, tt, tid, live, io, unused);
+                                                    lim, tt, tid, live, io);
     }
     // radices above EMAX are never scheduled: the planner checks radix <= emax before launch
 }
@@ -537,8 +497,9 @@ __device__ __forceinline__ int64_t logical_block(int remap)
 }
 
 // ------------------------------------------------------------------ shared kernel body
-template <int MAP>
-__device__ __forceinline__ void lane_map(const Geo& g, int& tt, int& tid)
+template <int MAP, int PRO, int EPI>
+__device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int& tt, int& tid, int64_t& blk, bool& live,
+                                             Io& io)
 {
     const int lane = threadIdx.x;
     if constexpr (MAP == kMapRows) {
@@ -548,12 +509,7 @@ __device__ __forceinline__ void lane_map(const Geo& g, int& tt, int& tid)
         tt = lane % g.tpb;
         tid = lane / g.tpb;
     }
-}
-
-// Pointers of transform blk*tpb + tt (tile blk of the launch).
-template <int MAP, int PRO, int EPI>
-__device__ __forceinline__ void tile_io(const FftArgs& a, const Geo& g, int64_t blk, int tt, bool& live, Io& io)
-{
+    blk = logical_block(a.xcd_remap);
     const int64_t t = blk * g.tpb + tt;
     live = t < a.count;
     io.in = a.in;
@@ -633,32 +589,40 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
 template <int EMAX>
 constexpr int max_block()
 {
-    return 512;
+    return EMAX <= 16 ? 256 : 512;
 }
 
-// Occupancy target: EMAX 16 kernels keep <= 128 VGPRs (4 waves per SIMD, two 512-thread
-// workgroups per CU); EMAX 32 kernels hold twice the points and run at 2 waves per SIMD.
+// Occupancy target of kernels that run EMAX-16 transforms in 512-thread workgroups: at
+// most 128 VGPRs (4 waves per SIMD); EMAX-32 kernels hold twice the points.
 template <int EMAX>
 constexpr int min_waves()
 {
     return EMAX <= 16 ? 4 : 1;
 }
 
+// Workgroup barrier for LDS traffic only: __syncthreads() also waits for every
+// outstanding global access (vmcnt 0); here only LDS accesses (lgkmcnt) are drained.  The
+// empty asm statements keep the compiler from moving LDS accesses across the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // ------------------------------------------------------------------ runtime kernel
 template <int EMAX, int MAP, int PRO, int EPI>
-__global__ void __launch_bounds__(max_block<EMAX>()) __attribute__((amdgpu_waves_per_eu(min_waves<EMAX>())))
-stockham_kernel(const FftArgs a)
+__global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftArgs a)
 {
     extern __shared__ float smem[];
     float* lre = smem;
     float* lim = smem + a.lds_plane;
     const Geo g{a.n, a.threads, a.tpb, a.lds_tstride};
     int tt, tid;
-    lane_map<MAP>(g, tt, tid);
-    const int64_t blk = logical_block(a.xcd_remap);
+    int64_t blk;
     bool live;
     Io io;
-    tile_io<MAP, PRO, EPI>(a, g, blk, tt, live, io);
+    thread_setup<MAP, PRO, EPI>(a, g, tt, tid, blk, live, io);
 
     for (int p = 0; p < a.passes; ++p) {
         // pass descriptors are uniform; readfirstlane keeps the switch a scalar branch
@@ -690,106 +654,37 @@ stockham_kernel(const FftArgs a)
 
 // ------------------------------------------------------------------ specialised kernel
 // Same passes with N, threads, transforms per workgroup and the radix sequence as
-// template constants (instances listed in static_instances.txt).  The kernel is
-// persistent: workgroup w processes tiles w, w + G, w + 2G, ... (G = gridDim.x) and,
-// for plain complex / real loads, gathers the next tile's first-pass inputs into
-// registers before running the current tile, so HBM reads overlap the LDS passes
-// and the stores of the previous tile.
-template <int EMAX, int MAP, int PRO, int EPI, bool PRE, bool TWL, int N, int T, int TPB, int P, int SPAN, int TWOFF,
-          int R, int... Rest>
+// template constants (instances listed in static_instances.h).
+template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
 __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
-                                              bool live, const Io& io, const float2 (&pre)[EMAX], const float2* twl)
+                                              bool live, const Io& io)
 {
-    stockham_pass<EMAX, R, MAP, PRO, EPI, false, PRE && P == 0, TWL>(a, g, SPAN, 0u, TWOFF, P == 0,
-                                                                     sizeof...(Rest) == 0, lre, lim, tt, tid, live,
-                                                                     io, pre, twl);
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt,
+                                                 tid, live, io);
     if constexpr (sizeof...(Rest) > 0) {
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
-        static_passes<EMAX, MAP, PRO, EPI, PRE, TWL, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(
-            a, g, lre, lim, tt, tid, live, io, pre, twl);
+        static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lre, lim, tt, tid,
+                                                                                         live, io);
     }
 }
 
-// Entries of the per-pass twiddle table of a radix sequence (pass p: span * (R - 1), none at span 1).
-template <int SPAN, int R, int... Rest>
-constexpr int twiddle_entries()
-{
-    constexpr int here = SPAN > 1 ? SPAN * (R - 1) : 0;
-    if constexpr (sizeof...(Rest) > 0) {
-        return here + twiddle_entries<SPAN * R, Rest...>();
-    } else {
-        return here;
-    }
-}
-
-// Persistent prefetching kernels stage the twiddle table in LDS when it is at most this large.
-constexpr int kTwiddleLdsMaxBytes = 48 * 1024;
-
-template <int EMAX, int MAP, int PRO, int R0, int... Rs>
-constexpr bool twiddles_in_lds()
-{
-    const bool prefetch = (PRO == kProComplex || PRO == kProReal) && EMAX <= 16 && MAP == kMapRows;
-    return prefetch && twiddle_entries<1, R0, Rs...>() * 8 <= kTwiddleLdsMaxBytes;
-}
-
-template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int R0, int... Rs>
-__global__ void __launch_bounds__(max_block<EMAX>()) __attribute__((amdgpu_waves_per_eu(min_waves<EMAX>())))
-stockham_static_kernel(const FftArgs a)
+template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int... Rs>
+__global__ void __launch_bounds__(max_block<EMAX>()) stockham_static_kernel(const FftArgs a)
 {
     extern __shared__ float smem[];
     constexpr int kTstride = N + (N >> 4) + 1;
-    // the prefetch buffer doubles the point registers: it fits without spills for rows at
-    // EMAX 16 (column tiles also keep per-point 64-bit strided addresses live)
-    constexpr bool kPrefetch = (PRO == kProComplex || PRO == kProReal) && EMAX <= 16 && MAP == kMapRows;
-    constexpr bool kTwl = twiddles_in_lds<EMAX, MAP, PRO, R0, Rs...>();
     float* lre = smem;
     float* lim = smem + a.lds_plane;
-    // twiddle table (when staged) follows the two data planes
-    float2* twl = reinterpret_cast<float2*>(smem + 2 * a.lds_plane);
-    if constexpr (kTwl) {
-        constexpr int kEntries = twiddle_entries<1, R0, Rs...>();
-        const float2* src = reinterpret_cast<const float2*>(a.tw);
-        for (int e = threadIdx.x; e < kEntries; e += blockDim.x) twl[e] = src[e];
-        // visible to other lanes from the first exchange barrier on (no pass before it reads twiddles)
-    }
     // rows mapping keeps tpb at run time (it only offsets the transform's LDS block);
     // the column mapping interleaves transforms inside LDS, so there it is a constant
     const Geo g{N, T, (MAP == kMapRows) ? a.tpb : TPB, kTstride};
     int tt, tid;
-    lane_map<MAP>(g, tt, tid);
-    const int64_t ntiles = (a.count + g.tpb - 1) / g.tpb;
-    const int64_t stride = gridDim.x;
-    int64_t tile = logical_block(a.xcd_remap);
+    int64_t blk;
     bool live;
     Io io;
-    tile_io<MAP, PRO, EPI>(a, g, tile, tt, live, io);
-    float2 cur[EMAX];
-    if constexpr (kPrefetch) load_first<EMAX, R0, MAP, PRO>(a, io, N, T, tid, live, cur);
-    while (tile < ntiles) {
-        const int64_t next = tile + stride;
-        bool live_n = false;
-        Io io_n = io;
-        float2 nxt[EMAX];
-        if constexpr (kPrefetch) {
-            if (next < ntiles) {
-                tile_io<MAP, PRO, EPI>(a, g, next, tt, live_n, io_n);
-                load_first<EMAX, R0, MAP, PRO>(a, io_n, N, T, tid, live_n, nxt);
-            }
-        }
-        static_passes<EMAX, MAP, PRO, EPI, kPrefetch, kTwl, N, T, TPB, 0, 1, 0, R0, Rs...>(a, g, lre, lim, tt, tid,
-                                                                                           live, io, cur, twl);
-        lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, tile, live, io);
-        tile = next;
-        if (tile >= ntiles) break;
-        if constexpr (kPrefetch) {
-            io = io_n;
-            live = live_n;
-#pragma unroll
-            for (int e = 0; e < EMAX; ++e) cur[e] = nxt[e];
-        } else {
-            tile_io<MAP, PRO, EPI>(a, g, tile, tt, live, io);
-        }
-    }
+    thread_setup<MAP, PRO, EPI>(a, g, tt, tid, blk, live, io);
+    static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, lre, lim, tt, tid, live, io);
+    lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
 }
 
 // ------------------------------------------------------------------ launch helper
@@ -808,7 +703,7 @@ hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipS
 {
     if (s.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds_bytes);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)s.grid), dim3((unsigned)s.block), s.lds_bytes, stream, args);
@@ -822,51 +717,11 @@ hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t
     return launch_kernel(stockham_kernel<EMAX, Tr::map, Tr::pro, Tr::epi>, s, args, stream);
 }
 
-// Persistent grid: at most `tiles_per_block` tiles per workgroup beyond what fits on the
-// device at once (workgroups per CU from the occupancy calculator x CUs).
-template <typename K>
-int64_t persistent_grid(K kern, const LaunchShape& s)
-{
-    static int cus = -1;
-    if (cus < 0) {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
-                                                     hipSuccess && n > 0) {
-            cus = n;
-        } else {
-            cus = 256;
-        }
-    }
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, s.block, s.lds_bytes) != hipSuccess || per_cu < 1) {
-        per_cu = 1;
-    }
-    const int64_t resident = (int64_t)per_cu * cus;
-    int64_t tpw = (s.grid + resident - 1) / resident;  // tiles per workgroup so everything is resident
-    if (s.tiles_per_block > tpw) tpw = s.tiles_per_block;
-    if (tpw < 1) tpw = 1;
-    return (s.grid + tpw - 1) / tpw;
-}
-
 template <int V, int EMAX, int N, int T, int TPB, int... Rs>
 hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     using Tr = VariantTraits<V>;
-    auto kern = stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>;
-    if constexpr (twiddles_in_lds<EMAX, Tr::map, Tr::pro, Rs...>()) {
-        // the kernel stages its twiddle table after the two data planes: refuse a launch
-        // whose dynamic LDS does not cover it
-        const size_t need = (size_t)8 * (size_t)args.lds_plane + (size_t)8 * twiddle_entries<1, Rs...>();
-        if (s.lds_bytes < need) return hipErrorInvalidValue;
-    }
-    LaunchShape p = s;
-    if (s.lds_bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds_bytes);
-        if (e != hipSuccess) return e;
-    }
-    p.grid = persistent_grid(kern, s);
-    return launch_kernel(kern, p, args, stream);
+    return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args, stream);
 }
 
 }  // namespace asdfft
