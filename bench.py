@@ -253,6 +253,82 @@ def bench_ragged(torch, dev, stream, reps, warmup):
     return res
 
 
+def serve_workload(seed=11, count=3000):
+    """Serving requests: 85% 1D (half C2C with N log-uniform over 5-smooth lengths in
+    [256, 65536], half R2C with N in [256, 32768]), batch uniform in 1..128; 15% 2D C2C
+    with X, Y in {64, ..., 1024}, batch 1.  Returns [(kind, dims, batch)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lens = np.array(smooth_lengths(256, 65536), dtype=np.int64)
+    logs = np.log(lens.astype(np.float64))
+    out = []
+    for _ in range(count):
+        if rng.uniform() < 0.85:
+            kind = "c2c" if rng.uniform() < 0.5 else "r2c"
+            hi = 65536 if kind == "c2c" else 32768
+            u = rng.uniform(math.log(256), math.log(hi))
+            n = int(lens[np.abs(logs - u).argmin()])
+            out.append((kind, (n,), int(rng.integers(1, 129))))
+        else:
+            x, y = (int(v) for v in rng.choice([64, 128, 256, 512, 1024], 2))
+            out.append(("c2c", (x, y), 1))
+    return out
+
+
+def bench_serve(torch, dev, rates=(5000, 20000, 50000), count=3000):
+    """Serving form: a Poisson request stream replayed by the library's native submitter,
+    once through the continuous-batching server and once as per-request plain execs (the
+    same server with batching off), under an 8 GiB HBM budget.  Latency = completion of a
+    request's launch group minus its scheduled arrival."""
+    import numpy as np
+
+    from sip_amd.fft import Server
+    work = serve_workload(count=count)
+
+    def nbytes(kind, dims, batch):
+        pts = math.prod(dims[:-1]) * batch
+        h = dims[-1] // 2 + 1
+        return {"c2c": (8 * pts * dims[-1], 8 * pts * dims[-1]),
+                "r2c": (4 * pts * dims[-1], 8 * pts * h)}[kind]
+
+    max_in = max(nbytes(*w)[0] for w in work)
+    src = (torch.rand(max_in // 4 + 64, device=dev) * 2 - 1)
+    arena_bytes = 3 << 30
+    arena = torch.empty(arena_bytes, dtype=torch.uint8, device=dev)
+    reqs, off = [], 0
+    for kind, dims, batch in work:
+        ob = nbytes(kind, dims, batch)[1]
+        if off + ob > arena_bytes:
+            off = 0
+        reqs.append(Server.request(kind, dims, batch, False, src.data_ptr(), arena.data_ptr() + off))
+        off = (off + ob + 255) // 256 * 256
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(12)
+    total_flop = sum(5.0 * math.prod(d) * math.log2(math.prod(d)) * b for _, d, b in work)
+    res = {"requests": count, "mix": "85% 1D C2C/R2C N 256..65536 b1..128, 15% 2D C2C <=1024^2",
+           "hbm_budget_bytes": 8 << 30, "rates": {}}
+    for batching in (True, False):
+        srv = Server(hbm_budget=8 << 30, slots=3, batching=batching)
+        try:
+            srv.replay(reqs, [0.0] * len(reqs))  # plans and resident classes built before timing
+            for rate in rates:
+                arrivals = np.cumsum(rng.exponential(1e6 / rate, count))
+                lat = np.array(srv.replay(reqs, arrivals.tolist()))
+                done = float((arrivals + lat).max())
+                entry = {"p50_us": round(float(np.percentile(lat, 50)), 1),
+                         "p99_us": round(float(np.percentile(lat, 99)), 1),
+                         "mean_us": round(float(lat.mean()), 1),
+                         "achieved_req_per_s": round(count / done * 1e6, 1),
+                         "gflops": round(total_flop / done / 1e3, 1)}
+                res["rates"].setdefault(str(rate), {})["batched" if batching else "per_request"] = entry
+            st = srv.stats()
+            res["batches_" + ("batched" if batching else "per_request")] = st["batches"]
+        finally:
+            srv.close()
+    del arena, src
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +340,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch form")
     ap.add_argument("--ragged-only", action="store_true", help="run only the ragged-batch form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
@@ -280,6 +358,9 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     if a.ragged_only:
         print(json.dumps({"forms": {"ragged": bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)}}))
+        return
+    if a.serve_only:
+        print(json.dumps({"forms": {"serving": bench_serve(torch, dev)}}))
         return
     L = N.lib()
     entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
@@ -384,8 +465,13 @@ def main():
         torch.cuda.synchronize()
         line["calibration_copy_bytes"] = {"read": src.numel() * 4, "write": dst.numel() * 4}
         del src, dst
+    forms = {}
     if not a.no_ragged:
-        line["forms"] = {"ragged": bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)}
+        forms["ragged"] = bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)
+    if not a.no_serve:
+        forms["serving"] = bench_serve(torch, dev)
+    if forms:
+        line["forms"] = forms
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     for j in jobs:

@@ -52,7 +52,6 @@ struct PoolClass {
 
 struct PlainPlan {
     BuiltPlan built;
-    std::vector<void*> workspace;  // one per slot (execs on different slots overlap)
 };
 
 struct Slot {
@@ -63,6 +62,10 @@ struct Slot {
     int* d_queue = nullptr;
     RaggedTile* h_tiles = nullptr;  // pinned staging
     int64_t* h_offsets = nullptr;
+    // plain plans of a batch run one after another on the slot's stream: one workspace,
+    // grown to the largest plan seen
+    void* workspace = nullptr;
+    size_t workspace_bytes = 0;
     std::vector<uint64_t> tickets;
     size_t bytes = 0;
     bool busy = false;
@@ -222,11 +225,7 @@ Server::~Server()
         if (s->d_queue) (void)hipFree(s->d_queue);
         if (s->h_tiles) (void)hipHostFree(s->h_tiles);
         if (s->h_offsets) (void)hipHostFree(s->h_offsets);
-    }
-    for (auto& kv : plans) {
-        for (void* w : kv.second->workspace) {
-            if (w) (void)hipFree(w);
-        }
+        if (s->workspace) (void)hipFree(s->workspace);
     }
     if (d_pool) (void)hipFree(d_pool);
     if (setup) (void)hipStreamDestroy(setup);
@@ -297,17 +296,6 @@ PlainPlan* Server::plain_plan(const Req& q, std::string* why)
             *why = "plan tables could not be uploaded";
             return nullptr;
         }
-    }
-    p->workspace.assign(slots.size(), nullptr);
-    if (p->built.workspace > 0) {
-        for (auto& w : p->workspace) {
-            if (hipMalloc(&w, p->built.workspace) != hipSuccess) {
-                *why = "workspace allocation failed";
-                return nullptr;
-            }
-        }
-        std::lock_guard<std::mutex> lock(mu);
-        resident += p->built.workspace * slots.size();
     }
     PlainPlan* raw = p.get();
     plans[q.plain_key] = std::move(p);
@@ -392,12 +380,37 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
         }
     }
     // ---- plain part: cached plans, launched on this slot's stream
+    std::vector<PlainPlan*> pplans;
+    size_t ws_need = 0;
     for (Req* q : plain) {
         std::string why;
         PlainPlan* p = plain_plan(*q, &why);
         if (p == nullptr) return 3;
+        pplans.push_back(p);
+        ws_need = std::max(ws_need, p->built.workspace);
+    }
+    if (ws_need > s.workspace_bytes) {
+        // the slot is idle (its previous batch completed), so its workspace is free to replace
+        if (s.workspace) (void)hipFree(s.workspace);
+        s.workspace = nullptr;
+        const size_t old_bytes = s.workspace_bytes;
+        s.workspace_bytes = 0;
+        if (hipMalloc(&s.workspace, ws_need) != hipSuccess) {
+            s.workspace = nullptr;
+            std::lock_guard<std::mutex> lock(mu);
+            resident -= old_bytes;
+            return 3;
+        }
+        s.workspace_bytes = ws_need;
+        std::lock_guard<std::mutex> lock(mu);
+        resident += ws_need - old_bytes;
+    }
+    (void)slot_index;
+    for (size_t k = 0; k < plain.size(); k++) {
+        Req* q = plain[k];
+        PlainPlan* p = pplans[k];
         char* bufs[3] = {static_cast<char*>(const_cast<void*>(q->r.in)), static_cast<char*>(q->r.out),
-                         static_cast<char*>(p->workspace[slot_index])};
+                         static_cast<char*>(s.workspace)};
         for (const Step& st : p->built.steps) {
             hipError_t e;
             if (st.direct) {
@@ -443,7 +456,10 @@ void Server::scheduler_loop()
         const size_t budget = cfg.hbm_budget - std::min(cfg.hbm_budget, resident);
         while (!pending.empty() && (int)batch.size() < cfg.max_batch) {
             Req& q = reqs[pending.front()];
-            if (inflight + add + q.bytes > budget) break;
+            // with nothing in flight the head request is admitted even if resident state
+            // (grown workspaces) has eaten into the budget since it was submitted
+            const bool idle = inflight == 0 && batch.empty();
+            if (inflight + add + q.bytes > budget && !idle) break;
             if (!cfg.batching && !batch.empty()) break;
             if (q.ragged && transforms + q.entry.count > kSlotTransforms) break;
             if (q.ragged) transforms += q.entry.count;
@@ -575,9 +591,14 @@ int serve_submit(Server* s, const ServeRequest& r, uint64_t* ticket, std::string
                  (long long)(r.rank > 1 ? r.dims[1] : 0), (long long)r.batch);
         q.plain_key = key;
         std::string w;
-        if (s->plain_plan(q, &w) == nullptr) {
+        PlainPlan* p = s->plain_plan(q, &w);
+        if (p == nullptr) {
             *why = w;
             return 2;
+        }
+        if (p->built.workspace > s->cfg.hbm_budget / 4) {
+            *why = "plan workspace larger than a quarter of the HBM budget";
+            return 1;
         }
     }
     std::lock_guard<std::mutex> lock(s->mu);

@@ -244,6 +244,71 @@ class RaggedPlan:
             pass
 
 
+class Server:
+    """Serving loop (asdFftExtServe*): continuous batching of FFT requests under an HBM budget.
+
+    Requests are (kind, dims, batch, inverse, input tensor, output tensor); dims has one
+    (1D) or two (2D) sizes, the real length last for r2c / c2r.
+    """
+
+    def __init__(self, hbm_budget: int = 8 << 30, slots: int = 3, max_batch: int = 256, batching: bool = True):
+        self.L = N.lib()
+        cfg = N.ServeConfig(hbm_budget, slots, max_batch, 1 if batching else 0, 0)
+        h = ctypes.c_void_p()
+        N.check("asdFftExtServeCreate", self.L.asdFftExtServeCreate(ctypes.byref(h), ctypes.byref(cfg)))
+        self.handle = h
+
+    @staticmethod
+    def request(kind, dims, batch, inverse, x_ptr: int, y_ptr: int):
+        dims = tuple(int(d) for d in dims)
+        r = N.ServeRequest()
+        r.type = _TYPE[kind]
+        r.direction = N.ASCEND_FFT_INVERSE if inverse else N.ASCEND_FFT_FORWARD
+        r.rank = len(dims)
+        r.dims[0] = dims[0]
+        r.dims[1] = dims[1] if len(dims) > 1 else 0
+        r.batch = int(batch)
+        r.input = x_ptr
+        r.output = y_ptr
+        return r
+
+    def submit(self, kind, dims, batch, inverse, x, y) -> int:
+        r = self.request(kind, dims, batch, inverse, x.data_ptr(), y.data_ptr())
+        t = ctypes.c_uint64(0)
+        N.check("asdFftExtServeSubmit", self.L.asdFftExtServeSubmit(self.handle, ctypes.byref(r), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_us: int = -1) -> float:
+        lat = ctypes.c_double(0)
+        N.check("asdFftExtServeWait", self.L.asdFftExtServeWait(self.handle, ticket, timeout_us, ctypes.byref(lat)))
+        return lat.value
+
+    def replay(self, requests, arrival_us):
+        """requests: list of N.ServeRequest; arrival_us: offsets from the call.  Returns latencies (us)."""
+        n = len(requests)
+        arr = (N.ServeRequest * n)(*requests)
+        t = (ctypes.c_double * n)(*[float(a) for a in arrival_us])
+        lat = (ctypes.c_double * n)()
+        N.check("asdFftExtServeReplay", self.L.asdFftExtServeReplay(self.handle, arr, t, n, lat))
+        return [lat[i] for i in range(n)]
+
+    def stats(self) -> dict:
+        c = N.ServeCounters()
+        N.check("asdFftExtServeStats", self.L.asdFftExtServeStats(self.handle, ctypes.byref(c)))
+        return {name: getattr(c, name) for name, _ in N.ServeCounters._fields_}
+
+    def close(self):
+        if self.handle is not None:
+            self.L.asdFftExtServeDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PlanCache:
     """LRU of plans keyed by (kind, dims, batch, inverse); the reference keeps 10."""
 

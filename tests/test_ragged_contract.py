@@ -181,3 +181,25 @@ def test_get_type_reports_first_entry(L):
         t = ctypes.c_int()
         assert L.asdFftGetType(h, ctypes.byref(t)) == OK
         assert t.value == R2C
+
+
+# ---- serving loop C ABI (no kernel launches) ------------------------------------------------
+def test_serve_create_guards(L):
+    h = ctypes.c_void_p()
+    assert L.asdFftExtServeCreate(None, None) == INVALID
+    bad = N.ServeConfig(0, 9, 0, 1, 0)  # more than 8 slots
+    assert L.asdFftExtServeCreate(ctypes.byref(h), ctypes.byref(bad)) == INVALID
+    if not has_gpu():
+        assert L.asdFftExtServeCreate(ctypes.byref(h), None) == INTERNAL
+        assert not h.value
+
+
+def test_serve_unknown_server(L):
+    fake = ctypes.c_void_p(0x1234)
+    r = N.ServeRequest()
+    t = ctypes.c_uint64()
+    assert L.asdFftExtServeSubmit(fake, ctypes.byref(r), ctypes.byref(t)) == INVALID
+    assert L.asdFftExtServeWait(fake, 1, 0, None) == INVALID
+    assert L.asdFftExtServeStats(fake, ctypes.byref(N.ServeCounters())) == INVALID
+    assert L.asdFftExtServeDestroy(fake) == INVALID
+    assert L.asdFftExtServeReplay(fake, None, None, 0, None) == INVALID
