@@ -75,6 +75,20 @@ int32_t asdFftExtExecRagged(void* handle, const void* in_base, void* out_base);
  * flop count 5 n log2 n summed over the transforms. */
 int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t* out_extent, double* bytes, double* flops);
 
+/* ---- precision form ------------------------------------------------------------------
+ * ASDFFT_PRECISION_SPLIT16 runs large-radix stages as matrix products on FP16 MFMA with
+ * split hi + lo operands (a b ~= a_hi b_hi + a_hi b_lo + a_lo b_hi) and FP32 accumulation,
+ * after a per-transform power-of-two scale.  A kernel exists for 1D C2C N = 1024 (two
+ * radix-32 stages); plans of other shapes keep the FP32 kernels.  Typical relative L2
+ * error vs FP64 is a few 1e-7 (the FP32 kernels: about 1e-7). */
+#define ASDFFT_PRECISION_FP32 0
+#define ASDFFT_PRECISION_SPLIT16 1
+
+/* No reference counterpart (precision form).  Chooses the arithmetic of the plan that the
+ * next asdFftMakePlan1D / 2D / 3D call builds on this handle; ACL_ERROR_INVALID_PARAM for an
+ * unknown mode or a handle whose plan already exists. */
+int32_t asdFftExtSetPrecision(void* handle, int32_t mode);
+
 /* ---- serving loop ------------------------------------------------------------------------
  * A server admits FFT requests in arrival order while their input + output bytes (plus a
  * plan workspace) fit an HBM budget, and runs them in batches: whenever one of `slots`

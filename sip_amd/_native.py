@@ -96,6 +96,8 @@ class ServeCounters(ctypes.Structure):
                 ("submitted", "completed", "batches", "ragged_requests", "plain_requests", "peak_inflight_bytes",
                  "resident_bytes")]
 
+ASDFFT_PRECISION_FP32 = 0
+ASDFFT_PRECISION_SPLIT16 = 1
 ASDFFT_RAGGED_GRAPH = 0x1
 ASDFFT_RAGGED_AUTOTUNE = 0x2
 
@@ -148,6 +150,7 @@ SIGNATURES = [
     ("asdFftExtExecRagged", _I32, [_VP, _VP, _VP]),
     ("asdFftExtRaggedInfo", _I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)]),
+    ("asdFftExtSetPrecision", _I32, [_VP, _I32]),
     ("asdFftExtServeCreate", _I32, [ctypes.POINTER(_VP), ctypes.POINTER(ServeConfig)]),
     ("asdFftExtServeSubmit", _I32, [_VP, ctypes.POINTER(ServeRequest), ctypes.POINTER(ctypes.c_uint64)]),
     ("asdFftExtServeWait", _I32, [_VP, ctypes.c_uint64, _I64, ctypes.POINTER(ctypes.c_double)]),

@@ -48,6 +48,7 @@ struct FftPlan {
     bool workspace_queried = false;
     asdfft::BuiltPlan built;
     std::unique_ptr<asdfft::RaggedPlan> ragged;  // set by asdFftExtMakePlanRagged
+    asdfft::Precision precision = asdfft::Precision::kFp32;  // asdFftExtSetPrecision
 };
 
 std::mutex g_table_mu;
@@ -100,6 +101,7 @@ AspbStatus finish_plan(FftPlan& p, const char* fn)
     spec.dims = p.sizes;
     spec.batch = p.vertical ? p.vertical_batch : p.batch;
     spec.vertical = p.vertical;
+    spec.precision = p.precision;
     asdfft::BuiltPlan built;
     std::string why;
     try {
@@ -241,7 +243,12 @@ AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output,
         if (only_step >= 0 && (size_t)only_step != si) continue;
         const asdfft::Step& s = bp.steps[si];
         hipError_t e;
-        if (s.direct) {
+        if (s.mfma) {
+            asdfft::MfmaFftArgs a = s.margs;
+            a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
+            a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
+            e = asdfft::launch_mfma_fft1024(a, s.mfma_grid, stream);
+        } else if (s.direct) {
             asdfft::DftArgs a = s.dargs;
             a.in = bufs[s.src] + s.src_offset;
             a.out = bufs[s.dst] + s.dst_offset;
@@ -794,5 +801,26 @@ extern "C" int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t
     if (out_extent != nullptr) *out_extent = (int64_t)r.out_extent;
     if (bytes != nullptr) *bytes = r.bytes;
     if (flops != nullptr) *flops = r.flops;
+    return ACL_SUCCESS;
+}
+
+extern "C" int32_t asdFftExtSetPrecision(void* handle, int32_t mode)
+{
+    const char* fn = "asdFftExtSetPrecision";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lock(sp->mu);
+    if (sp->state == PlanState::kInitialized) {
+        log_error(fn, "precision must be chosen before the plan is made");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (mode != ASDFFT_PRECISION_FP32 && mode != ASDFFT_PRECISION_SPLIT16) {
+        log_error(fn, "unknown precision mode");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    sp->precision = mode == ASDFFT_PRECISION_SPLIT16 ? asdfft::Precision::kSplit16 : asdfft::Precision::kFp32;
     return ACL_SUCCESS;
 }

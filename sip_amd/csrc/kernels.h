@@ -29,7 +29,7 @@ constexpr int kMaxDynamicLds = 160 * 1024;
 struct LaunchShape {
     int static_id = -1;  // index into kStaticInstances, or -1 for the run-time kernel
     int variant;
-    int emax;          // registers per thread (complex points): 16 or 32
+    int emax;          // registers per thread (complex points): 16, 32 or 48
     int block;         // threads per workgroup
     size_t lds_bytes;  // dynamic LDS
     int64_t grid;      // workgroups
@@ -57,4 +57,5 @@ hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStrea
 // (stockham_inst.hip built with -DASDFFT_VARIANT=v -DASDFFT_EMAX=e).
 #define ASDFFT_FOR_EACH_INSTANCE(X) \
     X(0, 16) X(1, 16) X(2, 16) X(3, 16) X(4, 16) X(5, 16) X(6, 16) X(7, 16) \
-    X(0, 32) X(1, 32) X(2, 32) X(3, 32) X(4, 32) X(5, 32) X(6, 32) X(7, 32)
+    X(0, 32) X(1, 32) X(2, 32) X(3, 32) X(4, 32) X(5, 32) X(6, 32) X(7, 32) \
+    X(0, 48) X(1, 48) X(2, 48) X(3, 48) X(4, 48) X(5, 48)

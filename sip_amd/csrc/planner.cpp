@@ -106,13 +106,30 @@ struct Config {
 // Rows mapping: lanes along a transform; several short transforms per workgroup.
 // want_tpb > 0 asks for that many transforms per workgroup (transposed stores need
 // >= 16 adjacent transforms for 128-byte runs) and may use the whole LDS.
+// The fewest-pass radix sequence, or, when its largest per-thread share does not fit
+// the register budgets, the first radix multiset (more passes) that does.
+std::vector<int> fitting_radices(int64_t n)
+{
+    std::vector<int> r = choose_radices(n);
+    if (r.empty()) return r;
+    for (int emax : {16, 32, 48}) {
+        if (threads_for(n, r, emax) > 0) return r;
+    }
+    for (const std::vector<int>& alt : radix_multisets(n, 3, 64)) {
+        for (int emax : {16, 32, 48}) {
+            if (threads_for(n, alt, emax) > 0) return alt;
+        }
+    }
+    return r;
+}
+
 bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, Config* c)
 {
-    std::vector<int> radices = choose_radices(n);
+    std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
     bool found = false;
     Config best;
-    for (int emax : {16, 32}) {
+    for (int emax : {16, 32, 48}) {
         const int T = threads_for(n, radices, emax);
         if (T == 0) continue;
         const int maxb = max_block_for(emax);
@@ -144,11 +161,11 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, Config* c
 // Column mapping: lanes across adjacent transforms (columns) for coalesced strided access.
 bool cols_config(int64_t n, int64_t count, Config* c)
 {
-    std::vector<int> radices = choose_radices(n);
+    std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
     bool found = false;
     for (int tpb : {16, 8, 4, 2, 1}) {
-        for (int emax : {16, 32}) {
+        for (int emax : {16, 32, 48}) {
             const int T = threads_for(n, radices, emax);
             if (T == 0 || tpb * T > max_block_for(emax)) continue;
             const int64_t plane = n * (tpb + 1);  // column layout: row pitch tpb + 1 words
@@ -494,6 +511,31 @@ bool real_rows(Kind kind, int64_t n, int64_t count, bool inverse, int src, int d
     return true;
 }
 
+// 1024-point C2C rows on the split-FP16 MFMA kernel (two radix-32 matmul stages).
+void mfma_rows_1024(int64_t count, bool inverse, BuiltPlan* bp)
+{
+    Step st;
+    st.mfma = true;
+    st.bytes = 16.0 * 1024.0 * (double)count;
+    TablePtr w32 = upload_table("roots:32", roots_table(32, 32));
+    TablePtr w1024 = upload_table("roots:1024", roots_table(1024, 1024));
+    st.tables_ready = w32 && w1024;
+    if (w32) st.tables.push_back(w32);
+    if (w1024) st.tables.push_back(w1024);
+    st.margs.w32 = w32 ? reinterpret_cast<const float2*>(w32->dev) : nullptr;
+    st.margs.tw1024 = w1024 ? reinterpret_cast<const float2*>(w1024->dev) : nullptr;
+    st.margs.count = count;
+    st.margs.conj_in = inverse;
+    st.margs.conj_out = inverse;
+    const int waves = mfma_fft1024_waves();
+    // two resident workgroups per CU (LDS and registers), all rows covered
+    st.mfma_grid = std::min<int64_t>((count + waves - 1) / waves, 2 * 256);
+    std::ostringstream os;
+    os << "mfma split-fp16 n=1024 count=" << count << " stages=32x32 grid=" << st.mfma_grid;
+    st.describe = os.str();
+    bp->steps.push_back(std::move(st));
+}
+
 }  // namespace
 
 int max_block_for(int emax) { return emax <= 16 ? 256 : 512; }
@@ -597,6 +639,10 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
                 return "";
             }
             if (!c2c_cols(n, B, B, 0, B, inv, kBufIn, kBufOut, out, &why)) return why;
+            return "";
+        }
+        if (spec.kind == Kind::C2C && spec.precision == Precision::kSplit16 && n == 1024) {
+            mfma_rows_1024(B, inv, out);
             return "";
         }
         if (spec.kind == Kind::C2C) {

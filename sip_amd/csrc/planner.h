@@ -16,6 +16,7 @@
 
 #include "fft_kernel_args.h"
 #include "kernels.h"
+#include "mfma_fft.h"
 #include "twiddles.h"
 
 namespace asdfft {
@@ -42,6 +43,10 @@ std::vector<int> choose_radices(int64_t n);
 // every radix is <= 16, then by butterfly cost.  At most `limit` of them.
 std::vector<std::vector<int>> radix_multisets(int64_t n, int extra, int limit);
 
+// choose_radices(n), or when that sequence needs more threads per transform than a
+// workgroup holds, the fewest-pass alternative multiset that fits.
+std::vector<int> fitting_radices(int64_t n);
+
 // Threads per transform needed to cover every pass with EMAX/R butterflies per
 // thread, or 0 when this EMAX cannot hold a radix of the sequence or the
 // transform does not fit the EMAX's workgroup limit.
@@ -51,8 +56,17 @@ int max_block_for(int emax);
 
 enum Buffer : int { kBufIn = 0, kBufOut = 1, kBufWork = 2 };
 
+// Arithmetic of a plan's kernels.
+enum class Precision {
+    kFp32 = 0,     // FP32 Stockham passes (default)
+    kSplit16 = 1,  // split-FP16 MFMA large-radix stages where a kernel exists (N = 1024 C2C rows)
+};
+
 struct Step {
     bool direct = false;
+    bool mfma = false;       // split-FP16 MFMA kernel (margs, mfma_grid)
+    MfmaFftArgs margs{};
+    int64_t mfma_grid = 0;
     LaunchShape shape{};
     FftArgs args{};
     DftArgs dargs{};
@@ -73,6 +87,7 @@ struct PlanSpec {
     std::vector<int64_t> dims;  // FFT sizes; the last one is the contiguous axis
     int64_t batch = 1;
     bool vertical = false;      // 1D VERTICAL layout: tensor (N, batch)
+    Precision precision = Precision::kFp32;
 };
 
 struct BuiltPlan {

@@ -22,7 +22,7 @@ namespace asdfft {
 
 namespace {
 
-constexpr size_t kLdsLimit = 160 * 1024;
+constexpr size_t kLdsLimit = kRaggedLdsLimit;
 constexpr size_t kLdsFloor = 64 * 1024;
 constexpr int64_t kMaxComplexLength = 16384;  // one class's transform fits one workgroup's LDS
 constexpr int kQueueStrideInts = 32;          // 128 bytes between launches' queue words
@@ -154,14 +154,31 @@ std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, boo
     if (prod != c.n || sch.radices.empty() || (int)sch.radices.size() > kMaxPasses) {
         return "schedule for " + c.key + " does not factor its length";
     }
-    c.emax = 0;
-    for (int emax : {16, 32}) {
-        const int t = min_threads(c.n, sch.radices, emax);
-        if (t > 0 && t <= kRaggedMaxBlock) {
-            c.emax = emax;
-            sch.threads = std::max(sch.threads, t);
-            break;
+    auto fit = [&](const std::vector<int>& r, int* emax_out, int* t_out) {
+        for (int emax : {16, 32, 48}) {
+            const int t = min_threads(c.n, r, emax);
+            if (t > 0 && t <= kRaggedMaxBlock) {
+                *emax_out = emax;
+                *t_out = t;
+                return true;
+            }
         }
+        return false;
+    };
+    c.emax = 0;
+    int emax = 0, t = 0;
+    if (!fit(sch.radices, &emax, &t) && fixed == nullptr) {
+        // the fewest passes need too many threads: take the first multiset that fits
+        for (const std::vector<int>& alt : radix_multisets(c.n, 3, 64)) {
+            if (fit(alt, &emax, &t)) {
+                sch.radices = alt;
+                break;
+            }
+        }
+    }
+    if (emax > 0) {
+        c.emax = emax;
+        sch.threads = std::max(sch.threads, t);
     }
     if (c.emax == 0 || sch.threads > kRaggedMaxBlock) {
         *unsupported = true;
@@ -304,7 +321,7 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
     }
 
     // ---- launches: one per register budget
-    for (int emax : {16, 32}) {
+    for (int emax : {16, 32, 48}) {
         RaggedLaunchPlan L;
         L.emax = emax;
         int maxT = 0;
@@ -316,7 +333,7 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
             one = std::max(one, class_lds(plan->classes[ci], 1));
         }
         if (L.class_ids.empty()) continue;
-        const int want = (emax == 16) ? opt.block16 : opt.block32;
+        const int want = (emax == 16) ? opt.block16 : (emax == 32 ? opt.block32 : 0);
         int block = want > 0 ? want : 512;
         block = std::max(block, (maxT + 63) / 64 * 64);
         block = std::min(block, kRaggedMaxBlock);
@@ -637,7 +654,7 @@ std::vector<ClassSchedule> class_candidates(int64_t n, int limit)
         std::vector<int> asc(ms.rbegin(), ms.rend());
         for (const std::vector<int>& order : {ms, asc}) {
             int t0 = 0;
-            for (int emax : {16, 32}) {
+            for (int emax : {16, 32, 48}) {
                 const int t = min_threads(n, order, emax);
                 if (t > 0 && t <= kRaggedMaxBlock) {
                     t0 = t;

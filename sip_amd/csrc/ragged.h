@@ -20,6 +20,9 @@ namespace asdfft {
 
 // Largest workgroup of the ragged kernel (its __launch_bounds__).
 constexpr int kRaggedMaxBlock = 512;
+// Dynamic LDS a ragged launch may take: the workgroup's 160 KiB less room for the
+// kernel's static tile index.
+constexpr size_t kRaggedLdsLimit = 160 * 1024 - 256;
 
 // ---- kernel side (ragged_kernel.hip) --------------------------------------------------
 const void* ragged_kernel_fn(int emax);

@@ -136,22 +136,31 @@ const void* kernel_fn()
 
 const void* ragged_kernel_fn(int emax)
 {
-    return emax <= 16 ? kernel_fn<16>() : kernel_fn<32>();
+    if (emax <= 16) return kernel_fn<16>();
+    return emax <= 32 ? kernel_fn<32>() : kernel_fn<48>();
 }
 
 hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu)
 {
     const void* fn = ragged_kernel_fn(emax);
     if (lds_bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
+        // the kernel's static LDS (the tile index) comes out of the same 160 KiB
+        hipFuncAttributes attr{};
+        hipError_t e = hipFuncGetAttributes(&attr, fn);
+        if (e != hipSuccess) return e;
+        const size_t limit = (size_t)kMaxDynamicLds - attr.sharedSizeBytes;
+        if (lds_bytes > limit) return hipErrorInvalidValue;
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
         if (e != hipSuccess) return e;
     }
     int n = 0;
     hipError_t e;
     if (emax <= 16) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<16>, block, lds_bytes);
-    } else {
+    } else if (emax <= 32) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<32>, block, lds_bytes);
+    } else {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<48>, block, lds_bytes);
     }
     if (e != hipSuccess) return e;
     *per_cu = n;
@@ -165,8 +174,11 @@ hipError_t launch_ragged(int emax, int64_t grid, int block, size_t lds_bytes, co
     if (emax <= 16) {
         hipLaunchKernelGGL(ragged_kernel<16>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
                            classes, tiles);
-    } else {
+    } else if (emax <= 32) {
         hipLaunchKernelGGL(ragged_kernel<32>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
+                           classes, tiles);
+    } else {
+        hipLaunchKernelGGL(ragged_kernel<48>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
                            classes, tiles);
     }
     return hipGetLastError();

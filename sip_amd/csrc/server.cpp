@@ -29,6 +29,7 @@ constexpr int kPoolClasses = 4096;           // resident ragged class descriptor
 constexpr int64_t kSlotTransforms = 1 << 16;  // ragged transforms per batch
 constexpr int kSlotTiles = 1 << 16;
 constexpr int kQueueStrideInts = 32;
+constexpr int kGroups = 3;                   // register budgets: EMAX 16, 32, 48
 constexpr size_t kServeLdsCap = 80 * 1024;   // per-class tile LDS (two workgroups per CU)
 constexpr int kBlock = 512;
 constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12;
@@ -158,9 +159,9 @@ bool Server::init(std::string* why)
         return false;
     }
     resident += sizeof(RaggedClass) * kPoolClasses;
-    for (int emax : {16, 32}) {
+    for (int emax : {16, 32, 48}) {
         int pc = 0;
-        if (ragged_prepare(emax, 160 * 1024, kBlock, &pc) != hipSuccess) {
+        if (ragged_prepare(emax, kRaggedLdsLimit, kBlock, &pc) != hipSuccess) {
             *why = "ragged kernel setup failed";
             return false;
         }
@@ -173,8 +174,9 @@ bool Server::init(std::string* why)
                   hipMalloc(reinterpret_cast<void**>(&s->d_tiles), sizeof(RaggedTile) * kSlotTiles) == hipSuccess &&
                   hipMalloc(reinterpret_cast<void**>(&s->d_offsets), sizeof(int64_t) * 2 * kSlotTransforms) ==
                       hipSuccess &&
-                  hipMalloc(reinterpret_cast<void**>(&s->d_queue), sizeof(int) * 2 * kQueueStrideInts) == hipSuccess &&
-                  hipMemset(s->d_queue, 0, sizeof(int) * 2 * kQueueStrideInts) == hipSuccess &&
+                  hipMalloc(reinterpret_cast<void**>(&s->d_queue), sizeof(int) * kGroups * kQueueStrideInts) ==
+                      hipSuccess &&
+                  hipMemset(s->d_queue, 0, sizeof(int) * kGroups * kQueueStrideInts) == hipSuccess &&
                   hipHostMalloc(reinterpret_cast<void**>(&s->h_tiles), sizeof(RaggedTile) * kSlotTiles, 0) ==
                       hipSuccess &&
                   hipHostMalloc(reinterpret_cast<void**>(&s->h_offsets), sizeof(int64_t) * 2 * kSlotTransforms, 0) ==
@@ -326,35 +328,36 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
     struct Costed {
         double cost;
         RaggedTile t;
-        int emax;
+        int group;
     };
+    auto group_of = [](int emax) { return emax <= 16 ? 0 : (emax <= 32 ? 1 : 2); };
     std::vector<Costed> tiles;
-    size_t lds[2] = {0, 0};
+    size_t lds[kGroups] = {0, 0, 0};
     for (auto& kv : per_class) {
         const PoolClass& pc = pool[kv.first];
         const int64_t count = (int64_t)kv.second.size() / 2;
         memcpy(s.h_offsets + 2 * nt, kv.second.data(), kv.second.size() * sizeof(int64_t));
         const double per = (double)pc.info.n * (double)(pc.info.sched.radices.size() + 1);
+        const int g = group_of(pc.info.emax);
         for (int64_t f = 0; f < count; f += pc.tpb) {
             RaggedTile t;
             t.cls = kv.first;
             t.count = (int)std::min<int64_t>(pc.tpb, count - f);
             t.first = nt + f;
-            tiles.push_back({per * t.count, t, pc.info.emax});
+            tiles.push_back({per * t.count, t, g});
         }
-        lds[pc.info.emax == 16 ? 0 : 1] = std::max(lds[pc.info.emax == 16 ? 0 : 1], pc.lds);
+        lds[g] = std::max(lds[g], pc.lds);
         nt += count;
     }
     std::stable_sort(tiles.begin(), tiles.end(), [](const Costed& a, const Costed& b) {
-        if (a.emax != b.emax) return a.emax < b.emax;
+        if (a.group != b.group) return a.group < b.group;
         return a.cost > b.cost;
     });
-    int64_t n16 = 0;
+    int64_t counts[kGroups] = {0, 0, 0};
     for (size_t i = 0; i < tiles.size(); i++) {
         s.h_tiles[i] = tiles[i].t;
-        if (tiles[i].emax == 16) n16++;
+        counts[tiles[i].group]++;
     }
-    const int64_t n32 = (int64_t)tiles.size() - n16;
     if (!tiles.empty()) {
         if (hipMemcpyAsync(s.d_offsets, s.h_offsets, sizeof(int64_t) * 2 * nt, hipMemcpyHostToDevice, s.stream) !=
                 hipSuccess ||
@@ -362,10 +365,11 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
                            s.stream) != hipSuccess) {
             return 3;
         }
-        for (int g = 0; g < 2; g++) {
-            const int64_t count = g == 0 ? n16 : n32;
+        int64_t first = 0;
+        for (int g = 0; g < kGroups; g++) {
+            const int64_t count = counts[g];
             if (count == 0) continue;
-            const int emax = g == 0 ? 16 : 32;
+            const int emax = 16 * (g + 1);
             RaggedArgs a{};
             a.in_base = nullptr;  // offsets hold absolute device addresses
             a.out_base = nullptr;
@@ -373,10 +377,10 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
             a.queue = s.d_queue + g * kQueueStrideInts;
             a.ntiles = (int)count;
             const int64_t grid = std::min<int64_t>(count, (int64_t)per_cu(emax, lds[g]) * cus);
-            if (launch_ragged(emax, grid, kBlock, lds[g], a, d_pool, s.d_tiles + (g == 0 ? 0 : n16), s.stream) !=
-                hipSuccess) {
+            if (launch_ragged(emax, grid, kBlock, lds[g], a, d_pool, s.d_tiles + first, s.stream) != hipSuccess) {
                 return 3;
             }
+            first += count;
         }
     }
     // ---- plain part: cached plans, launched on this slot's stream

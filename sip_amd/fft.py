@@ -26,7 +26,8 @@ _TYPE = {"c2c": N.ASCEND_FFT_C2C, "r2c": N.ASCEND_FFT_R2C, "c2r": N.ASCEND_FFT_C
 class Plan:
     """One plan handle.  dims: FFT sizes (1-3 of them); kind: 'c2c' | 'r2c' | 'c2r'."""
 
-    def __init__(self, kind: str, dims, batch: int, inverse: bool = False, vertical: bool = False):
+    def __init__(self, kind: str, dims, batch: int, inverse: bool = False, vertical: bool = False,
+                 precision: str = "fp32"):
         self.L = N.lib()
         self.kind = kind
         self.dims = tuple(int(d) for d in dims)
@@ -39,6 +40,9 @@ class Plan:
         direction = N.ASCEND_FFT_INVERSE if inverse else N.ASCEND_FFT_FORWARD
         t = _TYPE[kind]
         try:
+            if precision != "fp32":
+                mode = {"split16": N.ASDFFT_PRECISION_SPLIT16}[precision]
+                N.check("asdFftExtSetPrecision", self.L.asdFftExtSetPrecision(h, mode))
             if len(self.dims) == 1:
                 st = self.L.asdFftMakePlan1D(h, self.dims[0], t, direction, self.batch,
                                              N.ASCEND_FFT_VERTICAL if vertical else N.ASCEND_FFT_HORIZONTAL)

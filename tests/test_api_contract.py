@@ -294,3 +294,25 @@ def test_acl_tensor_accessors(L):
     assert L.aclGetDataType(t.ptr, ctypes.byref(dt)) == 0 and dt.value == N.ACL_COMPLEX64
     neg = (ctypes.c_int64 * 2)(3, -1)
     assert not L.aclCreateTensor(neg, 2, N.ACL_FLOAT, None, 0, N.ACL_FORMAT_ND, neg, 2, None)
+
+
+def test_set_precision_rules(L):
+    with Handle(L) as h:
+        assert L.asdFftExtSetPrecision(h, 7) == INVALID
+        assert L.asdFftExtSetPrecision(h, N.ASDFFT_PRECISION_SPLIT16) == OK
+        assert L.asdFftMakePlan1D(h, 1024, C2C, FWD, 8, H) == OK
+        need = ctypes.c_size_t()
+        L.asdFftExtDescribePlan(h, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(need.value)
+        L.asdFftExtDescribePlan(h, buf, len(buf), None)
+        assert "mfma split-fp16 n=1024" in buf.value.decode()
+        assert L.asdFftExtSetPrecision(h, N.ASDFFT_PRECISION_FP32) == INVALID  # plan already made
+    with Handle(L) as h:  # other lengths keep the FP32 kernels
+        assert L.asdFftExtSetPrecision(h, N.ASDFFT_PRECISION_SPLIT16) == OK
+        assert L.asdFftMakePlan1D(h, 2048, C2C, FWD, 8, H) == OK
+        need = ctypes.c_size_t()
+        L.asdFftExtDescribePlan(h, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(need.value)
+        L.asdFftExtDescribePlan(h, buf, len(buf), None)
+        assert "stockham" in buf.value.decode()
+    assert L.asdFftExtSetPrecision(ctypes.c_void_p(0x1234), 0) == INVALID

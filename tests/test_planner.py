@@ -83,3 +83,20 @@ def test_lds_within_gfx950_limit():
             if "stockham" in line:
                 assert int(field(line, "block")) <= 512, line
         p.close()
+
+
+def test_every_smooth_length_up_to_16384_has_a_plan():
+    """{2^a 3^b 5^c} lengths whose fewest-pass radix sequence needs more threads than a
+    workgroup holds (e.g. 14400, 15625) fall back to another multiset or a larger register
+    budget instead of failing."""
+    import bench
+    from sip_amd.fft import Plan, RaggedPlan
+    for n in bench.smooth_lengths(2, 16384):
+        for kind in ("c2c", "r2c", "c2r"):
+            Plan(kind, (n,), 2).close()
+    work = bench.ragged_workload()
+    p = RaggedPlan([(k, n, 1, False) for k, n in work])
+    try:
+        assert "4000 entries" in p.describe()
+    finally:
+        p.close()
