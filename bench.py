@@ -255,19 +255,21 @@ def bench_ragged(torch, dev, stream, reps, warmup):
 
 def serve_workload(seed=11, count=3000):
     """Serving requests: 85% 1D (half C2C with N log-uniform over 5-smooth lengths in
-    [256, 65536], half R2C with N in [256, 32768]), batch uniform in 1..128; 15% 2D C2C
-    with X, Y in {64, ..., 1024}, batch 1.  Returns [(kind, dims, batch)]."""
+    [256, 65536], half R2C with N in [256, 32768], odd real lengths at most 16384), batch
+    uniform in 1..128; 15% 2D C2C with X, Y in {64, ..., 1024}, batch 1.
+    Returns [(kind, dims, batch)]."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    lens = np.array(smooth_lengths(256, 65536), dtype=np.int64)
-    logs = np.log(lens.astype(np.float64))
+    lens = {"c2c": np.array(smooth_lengths(256, 65536), dtype=np.int64),
+            # odd real lengths above 16384 have no single-workgroup kernel
+            "r2c": np.array([n for n in smooth_lengths(256, 32768) if n <= 16384 or n % 2 == 0], dtype=np.int64)}
     out = []
     for _ in range(count):
         if rng.uniform() < 0.85:
             kind = "c2c" if rng.uniform() < 0.5 else "r2c"
-            hi = 65536 if kind == "c2c" else 32768
-            u = rng.uniform(math.log(256), math.log(hi))
-            n = int(lens[np.abs(logs - u).argmin()])
+            cand = lens[kind]
+            u = rng.uniform(math.log(256), math.log(cand[-1]))
+            n = int(cand[np.abs(np.log(cand.astype(np.float64)) - u).argmin()])
             out.append((kind, (n,), int(rng.integers(1, 129))))
         else:
             x, y = (int(v) for v in rng.choice([64, 128, 256, 512, 1024], 2))

@@ -37,7 +37,9 @@ HOST_SOURCES = ["acl_tensor.cpp", "twiddles.cpp", "planner.cpp", "ragged.cpp", "
                 "fft_api.cpp"]
 KERNEL_HEADERS = ["stockham_impl.h", "fft_kernel_args.h", "kernels.h", "radix_consts.h", "ragged.h", "mfma_fft.h"]
 # hand-written kernel translation units besides the generated instances
-KERNEL_SOURCES = ["dft_direct.hip", "ragged_kernel.hip", "mfma_fft.hip"]
+KERNEL_SOURCES = ["dft_direct.hip", "mfma_fft.hip"]
+# the ragged kernel builds once per register budget (one translation unit each)
+RAGGED_EMAX = [16, 32, 48]
 
 
 def instances():
@@ -140,6 +142,12 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
     objs.append(reg_obj)
     if force or _newer(reg_obj, [reg_src] + hdr):
         tasks.append(["g++"] + HOST_FLAGS + ["-c", reg_src, "-o", reg_obj])
+    for e in RAGGED_EMAX:
+        obj = os.path.join(BUILD, f"ragged_kernel_e{e}.o")
+        src = os.path.join(CSRC, "ragged_kernel.hip")
+        objs.append(obj)
+        if force or _newer(obj, [src] + hdr):
+            tasks.append(["hipcc"] + HIPCC_FLAGS + [f"-DASDFFT_RAGGED_EMAX={e}", "-c", src, "-o", obj])
     for f in KERNEL_SOURCES:
         obj = os.path.join(BUILD, f.replace(".hip", ".o"))
         objs.append(obj)

@@ -501,6 +501,11 @@ bool real_rows(Kind kind, int64_t n, int64_t count, bool inverse, int src, int d
             if (!stockham_step(n, kVarC2ROdd, false, io, inverse, 0, 0, &st, why)) return false;
         }
     } else {
+        if (n > kMaxDirect && !choose_radices(n).empty()) {
+            *why = "odd real length " + std::to_string(n) + " above " + std::to_string(kMaxSingleKernel) +
+                   " is not supported (one workgroup transforms at most that many complex points)";
+            return false;
+        }
         if (kind == Kind::R2C) {
             if (!direct_step(n, kProReal, kEpiHalf, count, n, h, 0, inverse, &st, why)) return false;
         } else {

@@ -29,6 +29,7 @@ constexpr int kQueueStrideInts = 32;          // 128 bytes between launches' que
 
 int64_t pad16(int64_t x) { return x + (x >> 4); }
 
+
 uint32_t magic_for(int span)
 {
     if (span <= 1) return 0;
@@ -90,6 +91,39 @@ void free_device(void* p)
 }
 
 }  // namespace
+
+int ragged_block_for(int emax) { return emax <= 16 ? 256 : 512; }
+
+// ---- dispatch over the per-budget kernel translation units
+#define ASDFFT_RAGGED_DECL(E)                                                                        \
+    const void* ragged_kernel_fn_e##E();                                                             \
+    hipError_t ragged_prepare_e##E(size_t lds_bytes, int block, int* per_cu);                         \
+    hipError_t launch_ragged_e##E(int64_t grid, int block, size_t lds_bytes, const RaggedArgs& args, \
+                                  const RaggedClass* classes, const RaggedTile* tiles, hipStream_t stream);
+ASDFFT_RAGGED_DECL(16)
+ASDFFT_RAGGED_DECL(32)
+ASDFFT_RAGGED_DECL(48)
+#undef ASDFFT_RAGGED_DECL
+
+const void* ragged_kernel_fn(int emax)
+{
+    if (emax <= 16) return ragged_kernel_fn_e16();
+    return emax <= 32 ? ragged_kernel_fn_e32() : ragged_kernel_fn_e48();
+}
+
+hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu)
+{
+    if (emax <= 16) return ragged_prepare_e16(lds_bytes, block, per_cu);
+    return emax <= 32 ? ragged_prepare_e32(lds_bytes, block, per_cu) : ragged_prepare_e48(lds_bytes, block, per_cu);
+}
+
+hipError_t launch_ragged(int emax, int64_t grid, int block, size_t lds_bytes, const RaggedArgs& args,
+                         const RaggedClass* classes, const RaggedTile* tiles, hipStream_t stream)
+{
+    if (emax <= 16) return launch_ragged_e16(grid, block, lds_bytes, args, classes, tiles, stream);
+    if (emax <= 32) return launch_ragged_e32(grid, block, lds_bytes, args, classes, tiles, stream);
+    return launch_ragged_e48(grid, block, lds_bytes, args, classes, tiles, stream);
+}
 
 int64_t ragged_in_bytes(const RaggedEntry& e)
 {
@@ -157,7 +191,7 @@ std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, boo
     auto fit = [&](const std::vector<int>& r, int* emax_out, int* t_out) {
         for (int emax : {16, 32, 48}) {
             const int t = min_threads(c.n, r, emax);
-            if (t > 0 && t <= kRaggedMaxBlock) {
+            if (t > 0 && t <= ragged_block_for(emax)) {
                 *emax_out = emax;
                 *t_out = t;
                 return true;
@@ -334,11 +368,12 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
         }
         if (L.class_ids.empty()) continue;
         const int want = (emax == 16) ? opt.block16 : (emax == 32 ? opt.block32 : 0);
-        int block = want > 0 ? want : 512;
+        int block = want > 0 ? want : ragged_block_for(emax);
         block = std::max(block, (maxT + 63) / 64 * 64);
         block = std::min(block, kRaggedMaxBlock);
         L.block = block;
-        const size_t budget = std::min(kLdsLimit, std::max(kLdsFloor, one));
+        // whole workgroups of transforms: tiles fill every lane that LDS allows
+        const size_t budget = std::min(kLdsLimit, std::max(kLdsFloor, one * (size_t)std::max(1, block / maxT)));
         for (int ci : L.class_ids) {
             RaggedClassInfo& c = plan->classes[ci];
             int tpb = block / c.sched.threads;

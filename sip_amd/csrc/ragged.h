@@ -114,6 +114,10 @@ struct RaggedPlan {
 };
 
 // ---- class helpers (shared with the serving loop) ------------------------------------
+// Default workgroup of a register budget: 256 threads hold EMAX-16 transforms of up to
+// 4096 points (four workgroups per CU at 34 KiB of LDS each); larger budgets use 512.
+int ragged_block_for(int emax);
+
 // Kernel variant, complex pass length, conjugation and load alignment of a class (the
 // same choices the batched planner makes for a plain 1D plan).
 struct ClassShape {

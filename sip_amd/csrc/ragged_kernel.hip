@@ -53,7 +53,7 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& r, const FftArgs& 
     for (int p = 0; p < passes; ++p) {
         switch (__builtin_amdgcn_readfirstlane(a.pass[p].radix)) {
 #define ASDFFT_CASE(RR) \
-    case RR: maybe_pass<EMAX, RR, kMapRows, PRO, EPI>(a, g, p, lre, lim, tt, tid, live, io); break;
+    case RR: maybe_pass<EMAX, RR, kMapRows, PRO, EPI, true>(a, g, p, lre, lim, tt, tid, live, io); break;
             ASDFFT_CASE(16)
             ASDFFT_CASE(12)
             ASDFFT_CASE(10)
@@ -83,12 +83,17 @@ ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const
 {
     extern __shared__ float smem[];
     __shared__ int s_tile;
+    // thread 0 takes the next tile index one tile ahead, so the atomic's round trip
+    // overlaps the current tile instead of sitting between tiles
+    int ahead = 0;
+    if (threadIdx.x == 0) ahead = atomicAdd(&r.queue[0], 1);
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&r.queue[0], 1);
+        if (threadIdx.x == 0) s_tile = ahead;
         lds_barrier();
         const int k = __builtin_amdgcn_readfirstlane(s_tile);
-        lds_barrier();  // every wave has read s_tile before thread 0 takes the next index
+        lds_barrier();  // every wave has read s_tile before thread 0 writes the next index
         if (k >= r.ntiles) break;
+        if (threadIdx.x == 0) ahead = atomicAdd(&r.queue[0], 1);
         const int cls = __builtin_amdgcn_readfirstlane(tiles[k].cls);
         const int count = __builtin_amdgcn_readfirstlane(tiles[k].count);
         const int64_t first = tiles[k].first;
@@ -112,8 +117,9 @@ ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const
             default:
                 break;
         }
-        // the next tile's first LDS write sits behind its first exchange barrier, which
-        // every wave reaches only after finishing this tile's LDS reads
+        // LDS reuse: the next tile's first LDS write sits behind its first exchange
+        // barrier, which every wave reaches only after finishing this tile's LDS reads; the
+        // epilogue's LDS reads are ordered by the two index barriers above
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -126,23 +132,24 @@ ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const
     }
 }
 
-template <int EMAX>
-const void* kernel_fn()
-{
-    return reinterpret_cast<const void*>(&ragged_kernel<EMAX>);
-}
-
 }  // namespace
 
-const void* ragged_kernel_fn(int emax)
+// Entry points of this translation unit's register budget (the build compiles this file
+// once per EMAX in {16, 32, 48}; ragged.cpp dispatches on the budget).
+#ifndef ASDFFT_RAGGED_EMAX
+#error "ASDFFT_RAGGED_EMAX must be defined"
+#endif
+#define ASDFFT_RNAME2(B, E) B##_e##E
+#define ASDFFT_RNAME(B, E) ASDFFT_RNAME2(B, E)
+
+const void* ASDFFT_RNAME(ragged_kernel_fn, ASDFFT_RAGGED_EMAX)()
 {
-    if (emax <= 16) return kernel_fn<16>();
-    return emax <= 32 ? kernel_fn<32>() : kernel_fn<48>();
+    return reinterpret_cast<const void*>(&ragged_kernel<ASDFFT_RAGGED_EMAX>);
 }
 
-hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu)
+hipError_t ASDFFT_RNAME(ragged_prepare, ASDFFT_RAGGED_EMAX)(size_t lds_bytes, int block, int* per_cu)
 {
-    const void* fn = ragged_kernel_fn(emax);
+    const void* fn = reinterpret_cast<const void*>(&ragged_kernel<ASDFFT_RAGGED_EMAX>);
     if (lds_bytes > 65536) {
         // the kernel's static LDS (the tile index) comes out of the same 160 KiB
         hipFuncAttributes attr{};
@@ -154,33 +161,20 @@ hipError_t ragged_prepare(int emax, size_t lds_bytes, int block, int* per_cu)
         if (e != hipSuccess) return e;
     }
     int n = 0;
-    hipError_t e;
-    if (emax <= 16) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<16>, block, lds_bytes);
-    } else if (emax <= 32) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<32>, block, lds_bytes);
-    } else {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<48>, block, lds_bytes);
-    }
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ragged_kernel<ASDFFT_RAGGED_EMAX>, block, lds_bytes);
     if (e != hipSuccess) return e;
     *per_cu = n;
     return hipSuccess;
 }
 
-hipError_t launch_ragged(int emax, int64_t grid, int block, size_t lds_bytes, const RaggedArgs& args,
-                         const RaggedClass* classes, const RaggedTile* tiles, hipStream_t stream)
+hipError_t ASDFFT_RNAME(launch_ragged, ASDFFT_RAGGED_EMAX)(int64_t grid, int block, size_t lds_bytes,
+                                                            const RaggedArgs& args, const RaggedClass* classes,
+                                                            const RaggedTile* tiles, hipStream_t stream)
 {
     if (block <= 0 || block > kRaggedMaxBlock || block % 64 != 0 || grid <= 0) return hipErrorInvalidValue;
-    if (emax <= 16) {
-        hipLaunchKernelGGL(ragged_kernel<16>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
-                           classes, tiles);
-    } else if (emax <= 32) {
-        hipLaunchKernelGGL(ragged_kernel<32>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
-                           classes, tiles);
-    } else {
-        hipLaunchKernelGGL(ragged_kernel<48>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes, stream, args,
-                           classes, tiles);
-    }
+    hipLaunchKernelGGL(ragged_kernel<ASDFFT_RAGGED_EMAX>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes,
+                       stream, args, classes, tiles);
     return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -31,7 +32,6 @@ constexpr int kSlotTiles = 1 << 16;
 constexpr int kQueueStrideInts = 32;
 constexpr int kGroups = 3;                   // register budgets: EMAX 16, 32, 48
 constexpr size_t kServeLdsCap = 80 * 1024;   // per-class tile LDS (two workgroups per CU)
-constexpr int kBlock = 512;
 constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12;
 
 struct Req {
@@ -161,7 +161,7 @@ bool Server::init(std::string* why)
     resident += sizeof(RaggedClass) * kPoolClasses;
     for (int emax : {16, 32, 48}) {
         int pc = 0;
-        if (ragged_prepare(emax, kRaggedLdsLimit, kBlock, &pc) != hipSuccess) {
+        if (ragged_prepare(emax, kRaggedLdsLimit, ragged_block_for(emax), &pc) != hipSuccess) {
             *why = "ragged kernel setup failed";
             return false;
         }
@@ -239,7 +239,7 @@ int Server::per_cu(int emax, size_t lds)
     auto it = occupancy.find({emax, lds});
     if (it != occupancy.end()) return it->second;
     int pc = 0;
-    if (ragged_prepare(emax, lds, kBlock, &pc) != hipSuccess || pc < 1) pc = 1;
+    if (ragged_prepare(emax, lds, ragged_block_for(emax), &pc) != hipSuccess || pc < 1) pc = 1;
     occupancy[{emax, lds}] = pc;
     return pc;
 }
@@ -264,7 +264,7 @@ int Server::pool_class(const RaggedEntry& e)
     if (!ragged_schedule(&pc.info, nullptr, &unsup).empty()) return -1;
     const size_t one = ragged_class_lds(pc.info, 1);
     const size_t cap = std::max(kServeLdsCap, one);
-    int tpb = std::max(1, kBlock / pc.info.sched.threads);
+    int tpb = std::max(1, ragged_block_for(pc.info.emax) / pc.info.sched.threads);
     while (tpb > 1 && ragged_class_lds(pc.info, tpb) > cap) tpb--;
     pc.tpb = tpb;
     pc.lds = ragged_class_lds(pc.info, tpb);
@@ -377,7 +377,8 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
             a.queue = s.d_queue + g * kQueueStrideInts;
             a.ntiles = (int)count;
             const int64_t grid = std::min<int64_t>(count, (int64_t)per_cu(emax, lds[g]) * cus);
-            if (launch_ragged(emax, grid, kBlock, lds[g], a, d_pool, s.d_tiles + first, s.stream) != hipSuccess) {
+            if (launch_ragged(emax, grid, ragged_block_for(emax), lds[g], a, d_pool, s.d_tiles + first, s.stream) !=
+                hipSuccess) {
                 return 3;
             }
             first += count;
@@ -661,9 +662,17 @@ int serve_replay(Server* s, const ServeRequest* reqs, const double* arrival_us, 
         while (Clock::now() < due[i]) {
         }
         std::string why;
-        if (serve_submit(s, reqs[i], &tickets[i], &why) != 0) {
+        const int st = serve_submit(s, reqs[i], &tickets[i], &why);
+        if (st != 0) {
             tickets[i] = 0;
-            worst = 1;
+            if (worst == 0) {
+                const char* q = getenv("ASDFFT_QUIET");
+                if (q == nullptr || q[0] != '1') {
+                    fprintf(stderr, "[asdfft] asdFftExtServeReplay: request %lld refused: %s\n", (long long)i,
+                            why.c_str());
+                }
+            }
+            worst = std::max(worst, st);
         }
     }
     for (int64_t i = 0; i < n; i++) {

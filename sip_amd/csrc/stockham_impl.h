@@ -352,11 +352,23 @@ __device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int
     }
 }
 
+// Workgroup barrier for LDS traffic only: __syncthreads() also waits for every
+// outstanding global access (vmcnt 0); here only LDS accesses (lgkmcnt) are drained.  The
+// empty asm statements keep the compiler from moving LDS accesses across the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // ------------------------------------------------------------------ one pass
 // RT: runtime geometry (the pass loop dispatches on the radix at run time).  The
 // opaque copies stop the compiler from hoisting every radix case's address
 // arithmetic out of that loop and keeping all of it live at once (hundreds of VGPRs).
-template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT>
+// LB: exchange barriers wait for LDS traffic only (lds_barrier), so a persistent
+// workgroup's stores of the previous tile keep draining while the next tile runs.
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false>
 __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
                                               bool first, bool last, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io)
@@ -433,7 +445,11 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 
     // 3. scatter
     if (!last || kEpiViaLds) {
-        __syncthreads();
+        if constexpr (LB) {
+            lds_barrier();
+        } else {
+            __syncthreads();
+        }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
@@ -450,7 +466,11 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 }
             }
         }
-        __syncthreads();
+        if constexpr (LB) {
+            lds_barrier();
+        } else {
+            __syncthreads();
+        }
     } else {
         if constexpr (EPI == kEpiComplex || EPI == kEpiHalf) {
             if (io.cout) {  // uniform branch
@@ -469,7 +489,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     }
 }
 
-template <int EMAX, int R, int MAP, int PRO, int EPI>
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool LB = false>
 __device__ __forceinline__ void maybe_pass(const FftArgs& a, const Geo& g, int p, float* lre, float* lim, int tt,
                                            int tid, bool live, const Io& io)
 {
@@ -477,8 +497,8 @@ __device__ __forceinline__ void maybe_pass(const FftArgs& a, const Geo& g, int p
         const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
         const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
         const int tw_offset = __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
-        stockham_pass<EMAX, R, MAP, PRO, EPI, true>(a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre,
-                                                    lim, tt, tid, live, io);
+        stockham_pass<EMAX, R, MAP, PRO, EPI, true, LB>(a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre,
+                                                        lim, tt, tid, live, io);
     }
     // radices above EMAX are never scheduled: the planner checks radix <= emax before launch
 }
@@ -600,15 +620,6 @@ constexpr int min_waves()
     return EMAX <= 16 ? 4 : 1;
 }
 
-// Workgroup barrier for LDS traffic only: __syncthreads() also waits for every
-// outstanding global access (vmcnt 0); here only LDS accesses (lgkmcnt) are drained.  The
-// empty asm statements keep the compiler from moving LDS accesses across the barrier.
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
 
 // ------------------------------------------------------------------ runtime kernel
 template <int EMAX, int MAP, int PRO, int EPI>
