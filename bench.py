@@ -76,6 +76,23 @@ def ragged_workload(seed=10, total=4000, c2c_frac=0.7):
     return [(str(k), int(n)) for k, n in zip(kinds, lens[idx])]
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stockham_plain_fp32_coverage_hbm_counters.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed counter summary (two rocprofv3 --pmc runs,
+    FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size): {"<workload> launch <i>": bytes}."""
+    out = {}
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes "):
+                key, val = line[len("traffic_bytes "):].rsplit(":", 1)
+                out[key.strip()] = int(val)
+    except OSError:
+        pass
+    return out
+
+
 def flops(dims, batch):
     n = math.prod(dims)
     return 5.0 * n * math.log2(n) * batch
@@ -428,10 +445,13 @@ def main():
     value = total_flop / ms_per_step / 1e6  # GFLOP/s
 
     dom = max(launches, key=lambda r: r["us"])
+    label = f"{dom['workload']} launch {dom['step']}"
+    traffic = measured_traffic().get(label)
     roofline = {"bound": "hbm", "achieved": round(dom["GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"{dom['workload']} launch {dom['step']}", "us": round(dom["us"], 2),
-                "algorithmic_bytes": dom["bytes"]}
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": label, "us": round(dom["us"], 2), "algorithmic_bytes": dom["bytes"]}
+    if traffic is not None:
+        roofline["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT) + " (FETCH_SIZE x2, WRITE_SIZE x1 per copy calibration)"
 
     line = {
         "metric": METRIC,
