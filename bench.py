@@ -193,6 +193,9 @@ def bench_ragged(torch, dev, stream, reps, warmup):
         p_stream = RaggedPlan(grouped)
         p_graph = RaggedPlan(grouped, graph=True)
         p_scat = RaggedPlan(scattered, graph=True)
+        t_tune = time.perf_counter()
+        p_tuned = RaggedPlan(grouped, graph=True, autotune=True)
+        res["autotune_plan_s"] = round(time.perf_counter() - t_tune, 2)
         g = torch.Generator(device=dev)
         g.manual_seed(10)
         xin = (torch.rand(p_stream.in_extent // 4, generator=g, device=dev) * 2 - 1).view(torch.uint8)
@@ -223,6 +226,7 @@ def bench_ragged(torch, dev, stream, reps, warmup):
             "ragged_stream": lambda: p_stream.execute(xin, yout, stream.cuda_stream),
             "ragged_graph": lambda: p_graph.execute(xin, yout, stream.cuda_stream),
             "ragged_graph_scattered": lambda: p_scat.execute(xs, ys, stream.cuda_stream),
+            "ragged_graph_autotuned": lambda: p_tuned.execute(xin, yout, stream.cuda_stream),
             "plain_per_size_loop": run_plain,
         }
         # the per-size loop captured once in a CUDA graph: the plain kernels without host overhead
@@ -258,14 +262,15 @@ def bench_ragged(torch, dev, stream, reps, warmup):
         res[name] = {"us": round(ms * 1e3, 2), "gflops": round(p_stream.flops / ms / 1e6, 1),
                      "hbm_frac": round(p_stream.algorithmic_bytes / ms / 1e6 / HBM_PEAK_GBS, 4)}
     best_plain = min(times["plain_per_size_loop"], times.get("plain_per_size_graph", float("inf")))
-    res["speedup_vs_best_plain"] = round(best_plain / times["ragged_graph"], 2)
+    best_ragged = min(times["ragged_graph"], times["ragged_graph_autotuned"])
+    res["speedup_vs_best_plain"] = round(best_plain / best_ragged, 2)
     res["plain_graph_captured"] = graph_ok
     res["launches"] = p_stream.describe().count("launch emax")
     for pl, tin, tout, fn in plain:
         tin.close()
         tout.close()
         pl.close()
-    for p in (p_stream, p_graph, p_scat):
+    for p in (p_stream, p_graph, p_scat, p_tuned):
         p.close()
     return res
 
@@ -294,7 +299,7 @@ def serve_workload(seed=11, count=3000):
     return out
 
 
-def bench_serve(torch, dev, rates=(5000, 20000, 50000), count=3000):
+def bench_serve(torch, dev, rates=(5000, 20000, 50000, 100000, 200000), count=3000):
     """Serving form: a Poisson request stream replayed by the library's native submitter,
     once through the continuous-batching server and once as per-request plain execs (the
     same server with batching off), under an 8 GiB HBM budget.  Latency = completion of a

@@ -94,7 +94,6 @@ struct RaggedArgs {
     const void* in_base;
     void* out_base;
     const int64_t* offsets;  // per transform: {input byte offset, output byte offset}
-    int* queue;              // {next tile, workgroups finished}: both zero between launches
     int ntiles;
 };
 

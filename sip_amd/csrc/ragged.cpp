@@ -5,7 +5,7 @@
 // EMAX register budget; classes with one budget share one persistent launch.  Every
 // class's transforms are listed in the offset table, cut into tiles of tpb transforms,
 // and the tiles of a launch are sorted by decreasing cost (longest-processing-time
-// first) so the dynamic tile queue ends with the cheapest tiles.
+// first), so the workgroups the hardware starts last carry the cheapest tiles.
 #include "ragged.h"
 
 #include <math.h>
@@ -25,7 +25,6 @@ namespace {
 constexpr size_t kLdsLimit = kRaggedLdsLimit;
 constexpr size_t kLdsFloor = 64 * 1024;
 constexpr int64_t kMaxComplexLength = 16384;  // one class's transform fits one workgroup's LDS
-constexpr int kQueueStrideInts = 32;          // 128 bytes between launches' queue words
 
 int64_t pad16(int64_t x) { return x + (x >> 4); }
 
@@ -271,12 +270,14 @@ RaggedPlan::~RaggedPlan()
     if (have && prev != device) (void)hipSetDevice(device);
     if (graph_exec != nullptr) (void)hipGraphExecDestroy(graph_exec);
     if (graph != nullptr) (void)hipGraphDestroy(graph);
+    for (hipStream_t st : aux) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
+    if (fork != nullptr) (void)hipEventDestroy(fork);
     for (RaggedLaunchPlan& l : launches) {
         free_device(l.d_classes);
         free_device(l.d_tiles);
     }
     free_device(d_offsets);
-    free_device(d_queue);
     if (have && prev != device) (void)hipSetDevice(prev);
 }
 
@@ -462,18 +463,16 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
         if (ragged_prepare(L.emax, L.lds_bytes, L.block, &per_cu) != hipSuccess || per_cu < 1) {
             return "the ragged kernel cannot run with " + std::to_string(L.lds_bytes) + " bytes of LDS";
         }
-        L.grid = std::min<int64_t>(L.ntiles, (int64_t)per_cu * device_cus());
+        // one workgroup per tile (opt.max_grid > 0 caps it: the workgroups then loop)
+        L.grid = opt.max_grid > 0 ? std::min<int64_t>(L.ntiles, opt.max_grid) : L.ntiles;
+        (void)device_cus;
     }
     const size_t obytes = plan->offsets.size() * sizeof(int64_t);
     if (hipMalloc(reinterpret_cast<void**>(&plan->d_offsets), obytes) != hipSuccess ||
         hipMemcpy(plan->d_offsets, plan->offsets.data(), obytes, hipMemcpyHostToDevice) != hipSuccess) {
         return "device allocation failed";
     }
-    const size_t qbytes = plan->launches.size() * kQueueStrideInts * sizeof(int);
-    if (hipMalloc(reinterpret_cast<void**>(&plan->d_queue), qbytes) != hipSuccess ||
-        hipMemset(plan->d_queue, 0, qbytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-        return "device allocation failed";
-    }
+    if (hipDeviceSynchronize() != hipSuccess) return "device error while uploading the plan";
     plan->ready = true;
     return "";
 }
@@ -484,7 +483,6 @@ RaggedArgs args_for(const RaggedPlan& p, size_t li, const void* in, void* out)
     a.in_base = in;
     a.out_base = out;
     a.offsets = p.d_offsets;
-    a.queue = p.d_queue + li * kQueueStrideInts;
     a.ntiles = (int)p.launches[li].ntiles;
     return a;
 }
@@ -511,10 +509,43 @@ hipError_t exec_ragged(RaggedPlan& p, const void* in, void* out, hipStream_t str
 {
     if (!p.ready) return hipErrorNotReady;
     if (!p.opt.graph) {
-        for (size_t li = 0; li < p.launches.size(); li++) {
-            const RaggedLaunchPlan& L = p.launches[li];
-            const RaggedArgs a = args_for(p, li, in, out);
-            hipError_t e = launch_ragged(L.emax, L.grid, L.block, L.lds_bytes, a, L.d_classes, L.d_tiles, stream);
+        const size_t nl = p.launches.size();
+        if (nl > 1 && p.aux.empty()) {
+            // the register budgets' launches run side by side on helper streams: a budget
+            // with few tiles (long transforms) would otherwise leave most CUs idle
+            for (size_t i = 0; i + 1 < nl; i++) {
+                hipStream_t st;
+                hipEvent_t ev;
+                if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return hipErrorOutOfMemory;
+                p.aux.push_back(st);
+                if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return hipErrorOutOfMemory;
+                p.joins.push_back(ev);
+            }
+            if (hipEventCreateWithFlags(&p.fork, hipEventDisableTiming) != hipSuccess) return hipErrorOutOfMemory;
+        }
+        hipError_t e;
+        if (nl > 1) {
+            e = hipEventRecord(p.fork, stream);
+            if (e != hipSuccess) return e;
+        }
+        // launch order: the budgets with the longest transforms (fewest, slowest tiles) first
+        for (size_t k = nl; k-- > 0;) {
+            const RaggedLaunchPlan& L = p.launches[k];
+            hipStream_t st = (k == 0) ? stream : p.aux[k - 1];
+            if (k > 0) {
+                e = hipStreamWaitEvent(st, p.fork, 0);
+                if (e != hipSuccess) return e;
+            }
+            const RaggedArgs a = args_for(p, k, in, out);
+            e = launch_ragged(L.emax, L.grid, L.block, L.lds_bytes, a, L.d_classes, L.d_tiles, st);
+            if (e != hipSuccess) return e;
+            if (k > 0) {
+                e = hipEventRecord(p.joins[k - 1], st);
+                if (e != hipSuccess) return e;
+            }
+        }
+        for (size_t k = 1; k < nl; k++) {
+            e = hipStreamWaitEvent(stream, p.joins[k - 1], 0);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -523,7 +554,7 @@ hipError_t exec_ragged(RaggedPlan& p, const void* in, void* out, hipStream_t str
     if (p.graph_exec == nullptr) {
         e = hipGraphCreate(&p.graph, 0);
         if (e != hipSuccess) return e;
-        // the launches touch disjoint outputs and queues: independent nodes
+        // the launches touch disjoint outputs: independent nodes
         for (size_t li = 0; li < p.launches.size(); li++) {
             RaggedArgs a = args_for(p, li, in, out);
             void* slots[3];

@@ -55,6 +55,7 @@ struct RaggedOptions {
     bool autotune = false;  // time candidate schedules on the device at plan time
     int block16 = 0;        // workgroup size of the EMAX-16 launch (0: automatic)
     int block32 = 0;        // workgroup size of the EMAX-32 launch (0: automatic)
+    int64_t max_grid = 0;   // workgroups per launch at most (0: one per tile)
     // schedules fixed by the caller or a previous tuning run, keyed by class key
     std::map<std::string, ClassSchedule> fixed;
 };
@@ -94,7 +95,6 @@ struct RaggedPlan {
     std::vector<RaggedLaunchPlan> launches;
     std::vector<int64_t> offsets;  // host copy of the offset table
     int64_t* d_offsets = nullptr;
-    int* d_queue = nullptr;        // two words per launch, 128 bytes apart
     std::vector<TablePtr> tables;
     int device = -1;
     bool ready = false;            // device tables uploaded
@@ -103,6 +103,10 @@ struct RaggedPlan {
     double bytes = 0;
     double flops = 0;              // 5 N log2 N per complex-equivalent transform
     RaggedOptions opt;
+    // stream exec: helper streams and events joining the register budgets' launches
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
     // graph replay state
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -3,11 +3,9 @@
 //
 // Work table: tiles[k] = {class, transforms in the tile, first offset-table entry};
 // classes[c] holds the Stockham pass list and geometry of one (kind, direction, length).
-// Workgroups take tile indices from queue[0] with an atomic increment until the table is
-// exhausted (tiles are sorted by decreasing cost on the host, so the largest go first and
-// the small ones fill the tail).  The last workgroup to leave sets both queue words back
-// to zero, so the next launch (or graph replay) starts from an empty queue without a
-// memset between launches.
+// Tiles are sorted by decreasing cost on the host, so the largest start first and the
+// small ones fill the tail.  The kernel keeps no state between launches: graph replays and
+// concurrent launches of one plan need no reset.
 #include "ragged.h"
 #include "stockham_impl.h"
 
@@ -82,18 +80,12 @@ __global__ void __launch_bounds__(kRaggedMaxBlock) __attribute__((amdgpu_waves_p
 ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const RaggedTile* __restrict__ tiles)
 {
     extern __shared__ float smem[];
-    __shared__ int s_tile;
-    // thread 0 takes the next tile index one tile ahead, so the atomic's round trip
-    // overlaps the current tile instead of sitting between tiles
-    int ahead = 0;
-    if (threadIdx.x == 0) ahead = atomicAdd(&r.queue[0], 1);
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = ahead;
-        lds_barrier();
-        const int k = __builtin_amdgcn_readfirstlane(s_tile);
-        lds_barrier();  // every wave has read s_tile before thread 0 writes the next index
-        if (k >= r.ntiles) break;
-        if (threadIdx.x == 0) ahead = atomicAdd(&r.queue[0], 1);
+    // Workgroup w runs tiles w, w + G, w + 2G, ... of the cost-sorted table (G = gridDim.x).
+    // With G = tiles (the default) every workgroup runs one tile and the hardware
+    // dispatcher, which starts workgroups in index order as CUs free up, balances the load;
+    // a smaller G makes the launch persistent.  No atomics: many workgroups incrementing one
+    // queue word serialise in L2.
+    for (int k = blockIdx.x; k < r.ntiles; k += gridDim.x) {
         const int cls = __builtin_amdgcn_readfirstlane(tiles[k].cls);
         const int count = __builtin_amdgcn_readfirstlane(tiles[k].count);
         const int64_t first = tiles[k].first;
@@ -117,18 +109,10 @@ ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const
             default:
                 break;
         }
-        // LDS reuse: the next tile's first LDS write sits behind its first exchange
-        // barrier, which every wave reaches only after finishing this tile's LDS reads; the
-        // epilogue's LDS reads are ordered by the two index barriers above
-    }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int done = atomicAdd(&r.queue[1], 1);
-        if (done == (int)gridDim.x - 1) {
-            // every other workgroup has taken its last index: reset for the next launch
-            atomicExch(&r.queue[0], 0);
-            atomicExch(&r.queue[1], 0);
-        }
+        // LDS reuse across tiles: the next tile's first LDS write sits behind its first
+        // exchange barrier, which every wave reaches only after this tile's LDS reads;
+        // a barrier here covers the epilogue reads of a tile followed by a single-pass one
+        lds_barrier();
     }
 }
 

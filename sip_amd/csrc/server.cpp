@@ -29,7 +29,6 @@ using Clock = std::chrono::steady_clock;
 constexpr int kPoolClasses = 4096;           // resident ragged class descriptors
 constexpr int64_t kSlotTransforms = 1 << 16;  // ragged transforms per batch
 constexpr int kSlotTiles = 1 << 16;
-constexpr int kQueueStrideInts = 32;
 constexpr int kGroups = 3;                   // register budgets: EMAX 16, 32, 48
 constexpr size_t kServeLdsCap = 80 * 1024;   // per-class tile LDS (two workgroups per CU)
 constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12;
@@ -60,7 +59,6 @@ struct Slot {
     hipEvent_t done = nullptr;
     RaggedTile* d_tiles = nullptr;
     int64_t* d_offsets = nullptr;
-    int* d_queue = nullptr;
     RaggedTile* h_tiles = nullptr;  // pinned staging
     int64_t* h_offsets = nullptr;
     // plain plans of a batch run one after another on the slot's stream: one workspace,
@@ -130,7 +128,6 @@ public:
     std::vector<TablePtr> tables;
     std::mutex plan_mu;  // plain plans are created by submitters and read by the scheduler
     std::map<std::string, std::unique_ptr<PlainPlan>> plans;
-    std::map<std::pair<int, size_t>, int> occupancy;  // (emax, lds) -> workgroups per CU
 
     ~Server();
     bool init(std::string* why);
@@ -139,7 +136,6 @@ public:
     int launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch);
     int pool_class(const RaggedEntry& e);
     PlainPlan* plain_plan(const Req& r, std::string* why);
-    int per_cu(int emax, size_t lds);
 };
 
 bool Server::init(std::string* why)
@@ -174,9 +170,6 @@ bool Server::init(std::string* why)
                   hipMalloc(reinterpret_cast<void**>(&s->d_tiles), sizeof(RaggedTile) * kSlotTiles) == hipSuccess &&
                   hipMalloc(reinterpret_cast<void**>(&s->d_offsets), sizeof(int64_t) * 2 * kSlotTransforms) ==
                       hipSuccess &&
-                  hipMalloc(reinterpret_cast<void**>(&s->d_queue), sizeof(int) * kGroups * kQueueStrideInts) ==
-                      hipSuccess &&
-                  hipMemset(s->d_queue, 0, sizeof(int) * kGroups * kQueueStrideInts) == hipSuccess &&
                   hipHostMalloc(reinterpret_cast<void**>(&s->h_tiles), sizeof(RaggedTile) * kSlotTiles, 0) ==
                       hipSuccess &&
                   hipHostMalloc(reinterpret_cast<void**>(&s->h_offsets), sizeof(int64_t) * 2 * kSlotTransforms, 0) ==
@@ -224,7 +217,6 @@ Server::~Server()
         if (s->done) (void)hipEventDestroy(s->done);
         if (s->d_tiles) (void)hipFree(s->d_tiles);
         if (s->d_offsets) (void)hipFree(s->d_offsets);
-        if (s->d_queue) (void)hipFree(s->d_queue);
         if (s->h_tiles) (void)hipHostFree(s->h_tiles);
         if (s->h_offsets) (void)hipHostFree(s->h_offsets);
         if (s->workspace) (void)hipFree(s->workspace);
@@ -232,16 +224,6 @@ Server::~Server()
     if (d_pool) (void)hipFree(d_pool);
     if (setup) (void)hipStreamDestroy(setup);
     if (have && prev != device) (void)hipSetDevice(prev);
-}
-
-int Server::per_cu(int emax, size_t lds)
-{
-    auto it = occupancy.find({emax, lds});
-    if (it != occupancy.end()) return it->second;
-    int pc = 0;
-    if (ragged_prepare(emax, lds, ragged_block_for(emax), &pc) != hipSuccess || pc < 1) pc = 1;
-    occupancy[{emax, lds}] = pc;
-    return pc;
 }
 
 // Index of the resident class for this entry (created and uploaded on first use), or -1.
@@ -374,9 +356,8 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
             a.in_base = nullptr;  // offsets hold absolute device addresses
             a.out_base = nullptr;
             a.offsets = s.d_offsets;
-            a.queue = s.d_queue + g * kQueueStrideInts;
             a.ntiles = (int)count;
-            const int64_t grid = std::min<int64_t>(count, (int64_t)per_cu(emax, lds[g]) * cus);
+            const int64_t grid = count;  // one workgroup per tile
             if (launch_ragged(emax, grid, ragged_block_for(emax), lds[g], a, d_pool, s.d_tiles + first, s.stream) !=
                 hipSuccess) {
                 return 3;
