@@ -29,11 +29,16 @@ using Clock = std::chrono::steady_clock;
 constexpr int kPoolClasses = 4096;           // resident ragged class descriptors
 constexpr int64_t kSlotTransforms = 1 << 16;  // ragged transforms per batch
 constexpr int kSlotTiles = 1 << 16;
+// 1D requests above this many bytes run as their own launch group (a batch's ragged
+// launch completes as a whole, so one large request would hold its small neighbours)
+constexpr size_t kRaggedMaxRequestBytes = (size_t)8 << 20;
+constexpr size_t kStageBytes = sizeof(int64_t) * 2 * kSlotTransforms + sizeof(RaggedTile) * kSlotTiles;
 constexpr int kGroups = 3;                   // register budgets: EMAX 16, 32, 48
 constexpr size_t kServeLdsCap = 80 * 1024;   // per-class tile LDS (two workgroups per CU)
 constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12;
 
 struct Req {
+    uint64_t ticket = 0;
     ServeRequest r;
     bool ragged = false;
     RaggedEntry entry;      // ragged: one entry of `batch` transforms
@@ -56,16 +61,20 @@ struct PlainPlan {
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
-    RaggedTile* d_tiles = nullptr;
-    int64_t* d_offsets = nullptr;
-    RaggedTile* h_tiles = nullptr;  // pinned staging
-    int64_t* h_offsets = nullptr;
+    // completion of a batch's launch groups, in launch order: the ragged launches, then
+    // each plain request; every group's requests complete as soon as its event fires
+    std::vector<hipEvent_t> events;
+    std::vector<std::vector<uint64_t>> groups;
+    std::vector<size_t> group_bytes;
+    std::vector<uint64_t> failed;  // tickets of a batch whose launches could not be enqueued
+    // a batch's work table: the offset table, then the tiles, staged in pinned memory and
+    // moved to the device with one copy
+    char* d_stage = nullptr;
+    char* h_stage = nullptr;
     // plain plans of a batch run one after another on the slot's stream: one workspace,
     // grown to the largest plan seen
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
-    std::vector<uint64_t> tickets;
     size_t bytes = 0;
     bool busy = false;
     bool launched = false;
@@ -121,7 +130,8 @@ public:
     std::vector<std::unique_ptr<Slot>> slots;
     std::thread scheduler;
     hipStream_t setup = nullptr;  // uploads of new resident state
-    // resident ragged classes (touched by the scheduler thread only, after creation)
+    // resident ragged classes, created at submission (pool_mu) and read by the scheduler
+    std::mutex pool_mu;
     RaggedClass* d_pool = nullptr;
     std::vector<PoolClass> pool;
     std::map<std::string, int> pool_index;
@@ -166,15 +176,12 @@ bool Server::init(std::string* why)
         auto s = std::make_unique<Slot>();
         // non-blocking streams: uploads of new classes on other streams never wait for batches
         bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
-                  hipEventCreateWithFlags(&s->done, hipEventDisableTiming) == hipSuccess &&
-                  hipMalloc(reinterpret_cast<void**>(&s->d_tiles), sizeof(RaggedTile) * kSlotTiles) == hipSuccess &&
-                  hipMalloc(reinterpret_cast<void**>(&s->d_offsets), sizeof(int64_t) * 2 * kSlotTransforms) ==
-                      hipSuccess &&
-                  hipHostMalloc(reinterpret_cast<void**>(&s->h_tiles), sizeof(RaggedTile) * kSlotTiles, 0) ==
-                      hipSuccess &&
-                  hipHostMalloc(reinterpret_cast<void**>(&s->h_offsets), sizeof(int64_t) * 2 * kSlotTransforms, 0) ==
-                      hipSuccess;
-        resident += sizeof(RaggedTile) * kSlotTiles + sizeof(int64_t) * 2 * kSlotTransforms;
+
+                  hipMalloc(reinterpret_cast<void**>(&s->d_stage), kStageBytes) == hipSuccess &&
+                  hipHostMalloc(reinterpret_cast<void**>(&s->h_stage), kStageBytes, 0) == hipSuccess;
+        resident += kStageBytes;
+        s->events.assign((size_t)cfg.max_batch + 1, nullptr);
+        for (auto& ev : s->events) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
         slots.push_back(std::move(s));
         if (!ok) {
             *why = "slot allocation failed";
@@ -214,11 +221,11 @@ Server::~Server()
     (void)hipDeviceSynchronize();
     for (auto& s : slots) {
         if (s->stream) (void)hipStreamDestroy(s->stream);
-        if (s->done) (void)hipEventDestroy(s->done);
-        if (s->d_tiles) (void)hipFree(s->d_tiles);
-        if (s->d_offsets) (void)hipFree(s->d_offsets);
-        if (s->h_tiles) (void)hipHostFree(s->h_tiles);
-        if (s->h_offsets) (void)hipHostFree(s->h_offsets);
+        for (hipEvent_t ev : s->events) {
+            if (ev) (void)hipEventDestroy(ev);
+        }
+        if (s->d_stage) (void)hipFree(s->d_stage);
+        if (s->h_stage) (void)hipHostFree(s->h_stage);
         if (s->workspace) (void)hipFree(s->workspace);
     }
     if (d_pool) (void)hipFree(d_pool);
@@ -229,6 +236,7 @@ Server::~Server()
 // Index of the resident class for this entry (created and uploaded on first use), or -1.
 int Server::pool_class(const RaggedEntry& e)
 {
+    std::lock_guard<std::mutex> lock(pool_mu);
     const std::string key = ragged_class_key(e);
     auto it = pool_index.find(key);
     if (it != pool_index.end()) return it->second;
@@ -291,12 +299,23 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
     // ---- ragged part: transforms grouped per resident class
     std::map<int, std::vector<int64_t>> per_class;  // pool index -> (in, out) address pairs
     std::vector<Req*> plain;
+    std::vector<uint64_t> ragged_tickets;
+    size_t ragged_bytes = 0;
+    s.groups.clear();
+    s.group_bytes.clear();
+    // the ragged launch pays a work-table copy: it runs when two or more small 1D requests
+    // share the batch; a lone request takes the plain exec path
+    int eligible = 0;
+    for (Req* q : batch) eligible += (q->ragged && q->bytes <= kRaggedMaxRequestBytes) ? 1 : 0;
     for (Req* q : batch) {
-        int ci = q->ragged ? pool_class(q->entry) : -1;
+        const bool take = eligible >= 2 && q->ragged && q->bytes <= kRaggedMaxRequestBytes;
+        int ci = take ? pool_class(q->entry) : -1;
         if (ci < 0) {
             plain.push_back(q);
             continue;
         }
+        ragged_tickets.push_back(q->ticket);
+        ragged_bytes += q->bytes;
         const int64_t ib = ragged_in_bytes(q->entry), ob = ragged_out_bytes(q->entry);
         auto& v = per_class[ci];
         const int64_t in0 = (int64_t)reinterpret_cast<uintptr_t>(q->r.in);
@@ -315,10 +334,16 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
     auto group_of = [](int emax) { return emax <= 16 ? 0 : (emax <= 32 ? 1 : 2); };
     std::vector<Costed> tiles;
     size_t lds[kGroups] = {0, 0, 0};
+    std::vector<PoolClass> used;  // copies: the pool vector may grow under other submitters
+    {
+        std::lock_guard<std::mutex> lock(pool_mu);
+        for (auto& kv : per_class) used.push_back(pool[kv.first]);
+    }
+    size_t ui = 0;
     for (auto& kv : per_class) {
-        const PoolClass& pc = pool[kv.first];
+        const PoolClass& pc = used[ui++];
         const int64_t count = (int64_t)kv.second.size() / 2;
-        memcpy(s.h_offsets + 2 * nt, kv.second.data(), kv.second.size() * sizeof(int64_t));
+        memcpy(reinterpret_cast<int64_t*>(s.h_stage) + 2 * nt, kv.second.data(), kv.second.size() * sizeof(int64_t));
         const double per = (double)pc.info.n * (double)(pc.info.sched.radices.size() + 1);
         const int g = group_of(pc.info.emax);
         for (int64_t f = 0; f < count; f += pc.tpb) {
@@ -336,14 +361,16 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
         return a.cost > b.cost;
     });
     int64_t counts[kGroups] = {0, 0, 0};
+    const size_t tiles_at = (size_t)nt * 2 * sizeof(int64_t);  // 16-byte aligned
+    RaggedTile* h_tiles = reinterpret_cast<RaggedTile*>(s.h_stage + tiles_at);
     for (size_t i = 0; i < tiles.size(); i++) {
-        s.h_tiles[i] = tiles[i].t;
+        h_tiles[i] = tiles[i].t;
         counts[tiles[i].group]++;
     }
+    const int64_t* d_offsets = reinterpret_cast<const int64_t*>(s.d_stage);
+    const RaggedTile* d_tiles = reinterpret_cast<const RaggedTile*>(s.d_stage + tiles_at);
     if (!tiles.empty()) {
-        if (hipMemcpyAsync(s.d_offsets, s.h_offsets, sizeof(int64_t) * 2 * nt, hipMemcpyHostToDevice, s.stream) !=
-                hipSuccess ||
-            hipMemcpyAsync(s.d_tiles, s.h_tiles, sizeof(RaggedTile) * tiles.size(), hipMemcpyHostToDevice,
+        if (hipMemcpyAsync(s.d_stage, s.h_stage, tiles_at + sizeof(RaggedTile) * tiles.size(), hipMemcpyHostToDevice,
                            s.stream) != hipSuccess) {
             return 3;
         }
@@ -355,15 +382,18 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
             RaggedArgs a{};
             a.in_base = nullptr;  // offsets hold absolute device addresses
             a.out_base = nullptr;
-            a.offsets = s.d_offsets;
+            a.offsets = d_offsets;
             a.ntiles = (int)count;
             const int64_t grid = count;  // one workgroup per tile
-            if (launch_ragged(emax, grid, ragged_block_for(emax), lds[g], a, d_pool, s.d_tiles + first, s.stream) !=
+            if (launch_ragged(emax, grid, ragged_block_for(emax), lds[g], a, d_pool, d_tiles + first, s.stream) !=
                 hipSuccess) {
                 return 3;
             }
             first += count;
         }
+        if (hipEventRecord(s.events[s.groups.size()], s.stream) != hipSuccess) return 3;
+        s.groups.push_back(ragged_tickets);
+        s.group_bytes.push_back(ragged_bytes);
     }
     // ---- plain part: cached plans, launched on this slot's stream
     std::vector<PlainPlan*> pplans;
@@ -412,8 +442,11 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
             }
             if (e != hipSuccess) return 3;
         }
+        if (hipEventRecord(s.events[s.groups.size()], s.stream) != hipSuccess) return 3;
+        s.groups.push_back({q->ticket});
+        s.group_bytes.push_back(q->bytes);
     }
-    return hipEventRecord(s.done, s.stream) == hipSuccess ? 0 : 3;
+    return 0;
 }
 
 void Server::scheduler_loop()
@@ -461,7 +494,6 @@ void Server::scheduler_loop()
         }
         Slot& s = *slots[si];
         s.busy = true;
-        s.tickets = tickets;
         s.bytes = add;
         inflight += add;
         stats.peak_inflight_bytes = std::max<uint64_t>(stats.peak_inflight_bytes, inflight);
@@ -470,6 +502,15 @@ void Server::scheduler_loop()
         const int st = launch_batch(s, si, batch);
         lk.lock();
         s.error = st;
+        s.failed.clear();
+        if (st != 0) {
+            // requests whose launch group was not enqueued complete with the error
+            for (uint64_t t : tickets) {
+                bool grouped = false;
+                for (const auto& g : s.groups) grouped = grouped || std::find(g.begin(), g.end(), t) != g.end();
+                if (!grouped) s.failed.push_back(t);
+            }
+        }
         s.launched = true;
         cv.notify_all();
     }
@@ -482,21 +523,32 @@ void Server::waiter_loop(Slot* s)
     for (;;) {
         cv.wait(lk, [&] { return stop || s->launched; });
         if (!s->launched) break;  // stopping with nothing in flight
-        lk.unlock();
-        int st = s->error;
-        if (hipEventSynchronize(s->done) != hipSuccess) st = 3;
-        const Clock::time_point now = Clock::now();
-        lk.lock();
-        for (uint64_t t : s->tickets) {
-            auto it = reqs.find(t);
-            if (it == reqs.end()) continue;
-            it->second.completed = now;
-            it->second.status = st;
-            stats.completed++;
+        auto finish = [&](const std::vector<uint64_t>& tickets, int st) {
+            const Clock::time_point now = Clock::now();
+            for (uint64_t t : tickets) {
+                auto it = reqs.find(t);
+                if (it == reqs.end()) continue;
+                it->second.completed = now;
+                it->second.status = st;
+                stats.completed++;
+            }
+        };
+        // groups complete in launch order (one stream); each releases its requests at once
+        for (size_t i = 0; i < s->groups.size(); i++) {
+            lk.unlock();
+            const int st = hipEventSynchronize(s->events[i]) == hipSuccess ? 0 : 3;
+            lk.lock();
+            finish(s->groups[i], st);
+            inflight -= s->group_bytes[i];
+            s->bytes -= s->group_bytes[i];
+            cv.notify_all();
         }
-        inflight -= s->bytes;
+        finish(s->failed, 3);
+        inflight -= s->bytes;  // the failed requests' bytes
         s->bytes = 0;
-        s->tickets.clear();
+        s->groups.clear();
+        s->group_bytes.clear();
+        s->failed.clear();
         s->launched = false;
         s->busy = false;
         cv.notify_all();
@@ -569,13 +621,15 @@ int serve_submit(Server* s, const ServeRequest& r, uint64_t* ticket, std::string
                    r.batch <= kSlotTransforms && reinterpret_cast<uintptr_t>(r.in) % cs.in_align == 0 &&
                    reinterpret_cast<uintptr_t>(r.out) % cs.out_align == 0;
     }
-    if (!q.ragged) {
-        // the plan is built (and its tables made resident) at submission, so a plan
-        // failure is reported here and never inside a batch
-        char key[160];
-        snprintf(key, sizeof(key), "%d:%d:%d:%lld:%lld:%lld", r.type, (int)r.inverse, r.rank, (long long)r.dims[0],
-                 (long long)(r.rank > 1 ? r.dims[1] : 0), (long long)r.batch);
-        q.plain_key = key;
+    char key[160];
+    snprintf(key, sizeof(key), "%d:%d:%d:%lld:%lld:%lld", r.type, (int)r.inverse, r.rank, (long long)r.dims[0],
+             (long long)(r.rank > 1 ? r.dims[1] : 0), (long long)r.batch);
+    q.plain_key = key;
+    if (q.ragged && s->pool_class(q.entry) < 0) q.ragged = false;  // class resident before any batch
+    {
+        // every request's plain plan is built (tables resident) at submission: a plan
+        // failure is reported here, and a ragged-eligible request that ends up alone in a
+        // batch runs through it without plan work on the scheduler thread
         std::string w;
         PlainPlan* p = s->plain_plan(q, &w);
         if (p == nullptr) {
@@ -594,6 +648,7 @@ int serve_submit(Server* s, const ServeRequest& r, uint64_t* ticket, std::string
         return 1;
     }
     const uint64_t t = s->next_ticket++;
+    q.ticket = t;
     s->reqs[t] = q;
     s->pending.push_back(t);
     s->stats.submitted++;
