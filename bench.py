@@ -353,6 +353,48 @@ def bench_serve(torch, dev, rates=(5000, 20000, 50000, 100000, 200000), count=30
     return res
 
 
+def bench_precision(torch, dev, stream, reps):
+    """Precision form: 1D C2C N=65536 batch 64 and N=1024 batch 8192, inputs uniform[-1,1]
+    scaled by 10^k (k uniform in {-3..3} per transform).  Times the FP32 kernels and, where a
+    kernel exists (N=1024), the split-FP16 MFMA kernel; reports the relative L2 error of each
+    against numpy's FP64 FFT on 64 sampled transforms."""
+    import numpy as np
+
+    from sip_amd.fft import Plan
+    res = {}
+    for n, batch, seed in ((65536, 64, 8), (1024, 8192, 9)):
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        x = torch.complex(torch.rand(batch, n, generator=g, device=dev) * 2 - 1,
+                          torch.rand(batch, n, generator=g, device=dev) * 2 - 1)
+        k = torch.randint(-3, 4, (batch, 1), generator=g, device=dev).to(torch.float32)
+        x = (x * torch.pow(10.0, k)).contiguous()
+        rows = np.random.default_rng(seed).choice(batch, 64, replace=False)
+        xs = x[torch.as_tensor(rows, device=dev)].cpu().numpy().astype(np.complex128)
+        gold = np.fft.fft(xs, axis=1)
+        entry = {}
+        for prec in ("fp32", "split16"):
+            with torch.cuda.stream(stream):
+                p = Plan("c2c", (n,), batch, False, precision=prec)
+                used = "mfma split-fp16" in p.describe()
+                if prec == "split16" and not used:
+                    entry[prec] = {"available": False}
+                    p.close()
+                    continue
+                y = torch.empty_like(x)
+                p.bind_workspace(dev)
+                p.bind_stream(stream.cuda_stream)
+                ms = event_time_ms(torch, stream, lambda: p.execute(x, y), reps)
+                got = y[torch.as_tensor(rows, device=dev)].cpu().numpy().astype(np.complex128)
+                err = np.linalg.norm(got - gold, axis=1) / np.linalg.norm(gold, axis=1)
+                entry[prec] = {"us": round(ms * 1e3, 2), "gflops": round(flops((n,), batch) / ms / 1e6, 1),
+                               "hbm_frac": round(16.0 * n * batch / ms / 1e6 / HBM_PEAK_GBS, 4),
+                               "rel_l2_max": float(f"{err.max():.3e}"), "rel_l2_rms": float(f"{np.sqrt((err**2).mean()):.3e}")}
+                p.close()
+        res[f"c2c_n{n}_b{batch}"] = entry
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +407,7 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch form")
     ap.add_argument("--ragged-only", action="store_true", help="run only the ragged-batch form")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision form")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
@@ -497,6 +540,8 @@ def main():
         forms["ragged"] = bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)
     if not a.no_serve:
         forms["serving"] = bench_serve(torch, dev)
+    if not a.no_precision:
+        forms["precision"] = bench_precision(torch, dev, stream, max(5, a.kernel_reps))
     if forms:
         line["forms"] = forms
     if not a.no_cpu:

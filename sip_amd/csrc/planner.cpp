@@ -1,6 +1,7 @@
 #include "planner.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 
 #include <algorithm>
@@ -163,6 +164,22 @@ bool cols_config(int64_t n, int64_t count, Config* c)
 {
     std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
+    // ASDFFT_COLS="n:emax:T:tpb" forces a column geometry that has a specialised kernel
+    // (measurement of alternatives)
+    if (const char* e = getenv("ASDFFT_COLS")) {
+        long fn = 0, fe = 0, ft = 0, fp = 0;
+        if (sscanf(e, "%ld:%ld:%ld:%ld", &fn, &fe, &ft, &fp) == 4 && fn == n &&
+            find_static(kVarColsC2C, (int)fe, (int)n, (int)ft, (int)fp, radices.data(), (int)radices.size()) >= 0) {
+            c->radices = radices;
+            c->emax = (int)fe;
+            c->threads = (int)ft;
+            c->tpb = (int)fp;
+            c->tstride = 0;
+            c->plane = n * (fp + 1);
+            c->lds = (size_t)(8 * c->plane);
+            return true;
+        }
+    }
     bool found = false;
     for (int tpb : {16, 8, 4, 2, 1}) {
         for (int emax : {16, 32, 48}) {

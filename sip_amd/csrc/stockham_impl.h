@@ -679,8 +679,16 @@ __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, fl
     }
 }
 
+// Column kernels know their workgroup (T x TPB) at compile time and take it as the launch
+// bound, so an EMAX-16 column tile may use 512 threads; row kernels keep tpb at run time.
+template <int EMAX, int MAP, int T, int TPB>
+constexpr int static_block_bound()
+{
+    return MAP == kMapCols ? T * TPB : max_block<EMAX>();
+}
+
 template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int... Rs>
-__global__ void __launch_bounds__(max_block<EMAX>()) stockham_static_kernel(const FftArgs a)
+__global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) stockham_static_kernel(const FftArgs a)
 {
     extern __shared__ float smem[];
     constexpr int kTstride = N + (N >> 4) + 1;
