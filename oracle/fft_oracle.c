@@ -317,3 +317,31 @@ double oracle_rel_l2(const float* got, const double* golden, long count)
     if (den == 0.0L) return (double)sqrtl(num);
     return (double)sqrtl(num / den);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Batched 2D circular shift (include/fft_api.h:97-122 contract of          */
+/* asdFftShift / asdIfftShift; goldens in                                   */
+/* tests/ut/unittest/fft/{fftshift,ifftshift}/*_data/gen_data.py):           */
+/*   out[s][(r + sr) % row][(c + sc) % col] = scale * in[s*dist + r*col + c] */
+/* with (sr, sc) = (ceil(row/2), ceil(col/2)) for the forward shift and     */
+/* (floor(row/2), floor(col/2)) for the inverse; the scale is applied in    */
+/* float32 (each of re, im multiplied by (float)scale) only when apply != 0.*/
+/* in/out hold interleaved (re, im) float pairs.                            */
+/* ------------------------------------------------------------------------- */
+void oracle_shift2d(const float* in, float* out, long row, long col, long batch, long dist, int inverse,
+                    double scale, int apply)
+{
+    const long sr = inverse ? row / 2 : (row + 1) / 2;
+    const long sc = inverse ? col / 2 : (col + 1) / 2;
+    const float f = (float)scale;
+    for (long s = 0; s < batch; s++) {
+        for (long r = 0; r < row; r++) {
+            for (long c = 0; c < col; c++) {
+                const float* src = in + 2 * (s * dist + r * col + c);
+                float* dst = out + 2 * (s * row * col + ((r + sr) % row) * col + (c + sc) % col);
+                dst[0] = apply ? src[0] * f : src[0];
+                dst[1] = apply ? src[1] * f : src[1];
+            }
+        }
+    }
+}

@@ -46,6 +46,8 @@ def _load():
             getattr(lib, name).argtypes = [fp, dp, ctypes.c_long, lp, ctypes.c_int, ctypes.c_int]
         lib.oracle_lcg_uniform.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_float, ctypes.c_float,
                                            fp, ctypes.c_long]
+        lib.oracle_shift2d.argtypes = [fp, fp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_int]
         lib.oracle_rel_l2.argtypes = [fp, dp, ctypes.c_long]
         lib.oracle_rel_l2.restype = ctypes.c_double
         _lib = lib
@@ -124,6 +126,19 @@ def r2c(x, inverse=False):
 
 def c2r(x, n, inverse=False):
     return c2r_nd(x, (n,), inverse)
+
+
+def shift2d(x: np.ndarray, row: int, col: int, batch: int, dist: int = 0, inverse: bool = False,
+            scale: float = 1.0) -> np.ndarray:
+    """asdFftShift (inverse=False) / asdIfftShift (inverse=True) on complex64 slices of a flat
+    input (slice s at element s*dist; dist <= 0 means row*col).  Returns complex64 (batch,row,col)."""
+    dist = row * col if dist <= 0 else dist
+    src = np.ascontiguousarray(x, dtype=np.complex64).reshape(-1)
+    out = np.empty((batch, row, col), dtype=np.complex64)
+    apply = 1 if (inverse and scale != 1.0) else 0
+    _load().oracle_shift2d(_ptr(src.view(np.float32), ctypes.c_float), _ptr(out.view(np.float32), ctypes.c_float),
+                           row, col, batch, dist, int(inverse), float(scale), apply)
+    return out
 
 
 def lcg_uniform(n: int, state: int = 2026, lower: float = -300.0, upper: float = 300.0):

@@ -35,9 +35,9 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFOR
               f"-I{ROCM}/include", f"-I{INCLUDE}", f"-I{CSRC}"]
 HOST_SOURCES = ["acl_tensor.cpp", "twiddles.cpp", "planner.cpp", "ragged.cpp", "server.cpp", "serve_api.cpp",
                 "fft_api.cpp"]
-KERNEL_HEADERS = ["stockham_impl.h", "fft_kernel_args.h", "kernels.h", "radix_consts.h", "ragged.h", "mfma_fft.h"]
+KERNEL_HEADERS = ["stockham_impl.h", "fft_kernel_args.h", "kernels.h", "radix_consts.h", "ragged.h", "mfma_fft.h", "shift.h"]
 # hand-written kernel translation units besides the generated instances
-KERNEL_SOURCES = ["dft_direct.hip", "mfma_fft.hip"]
+KERNEL_SOURCES = ["dft_direct.hip", "mfma_fft.hip", "shift_kernel.hip"]
 # the ragged kernel builds once per register budget (one translation unit each)
 RAGGED_EMAX = [16, 32, 48]
 

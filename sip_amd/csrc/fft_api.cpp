@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -23,6 +24,7 @@
 #include "asdfft_ext.h"
 #include "planner.h"
 #include "ragged.h"
+#include "shift.h"
 #include "tensor_desc.h"
 
 namespace {
@@ -303,6 +305,106 @@ AspbStatus exec_common(asdFftHandle handle, const aclTensor* input, const aclTen
         return ACL_ERROR_INVALID_PARAM;
     }
     return run_plan(p, input, output, fn, only_step);
+}
+
+// asdFftShift / asdIfftShift (core/fft/fftshift.cpp:101-138 and ifftshift.cpp): the
+// reference's parameter checks in its order, then one HBM-bound gather kernel.
+//   asdFftShift : out[s][(r + ceil(row/2)) % row][(c + ceil(col/2)) % col] = in[s*dist + r*col + c]
+//   asdIfftShift: out[s][(r + row/2) % row][(c + col/2) % col] = float(scale) * in[...]
+bool storage_numel(const aclTensor* t, int64_t* n)
+{
+    int64_t v = 1;
+    for (int64_t d : t->storage) {
+        if (d < 0 || (d > 0 && v > std::numeric_limits<int64_t>::max() / d)) return false;
+        v *= d;
+    }
+    *n = v;
+    return true;
+}
+
+AspbStatus run_shift(const char* fn, int64_t row, int64_t col, int64_t batch, int64_t dist, double scale,
+                     bool inverse, const aclTensor* input, aclTensor* output, void* stream)
+{
+    constexpr int64_t kMax = std::numeric_limits<int64_t>::max();
+    if (row <= 0 || col <= 0 || batch <= 0) {
+        log_error(fn, "row, col and batch must be positive");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (row > kMax / col) {
+        log_error(fn, "row * col overflows int64");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const int64_t slice = row * col;
+    if (dist <= 0) {
+        dist = slice;
+    } else if (dist < slice) {
+        log_error(fn, "dist must be at least row * col when positive");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (input == nullptr || output == nullptr) {
+        log_error(fn, "tensor is nullptr");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (input->dtype != ACL_COMPLEX64 || output->dtype != ACL_COMPLEX64) {
+        log_error(fn, "input and output must be complex64");
+        return ACL_ERROR_UNSUPPORTED_DATA_TYPE;
+    }
+    if (batch - 1 > (kMax - slice) / dist) {
+        log_error(fn, "(batch - 1) * dist + row * col overflows int64");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (batch > kMax / slice) {
+        log_error(fn, "batch * row * col overflows int64");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const int64_t need_in = (batch - 1) * dist + slice;
+    const int64_t need_out = batch * slice;
+    int64_t have_in = 0, have_out = 0;
+    if (!storage_numel(input, &have_in) || have_in < need_in) {
+        log_error(fn, "input holds fewer elements than the slices need");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!storage_numel(output, &have_out) || have_out < need_out) {
+        log_error(fn, "output holds fewer elements than the slices need");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    void* in = asdfft::data_address(input);
+    void* out = asdfft::data_address(output);
+    if (in == nullptr || out == nullptr) {
+        log_error(fn, "tensor data is nullptr");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    if (ranges_overlap(in, (size_t)need_in * 8, out, (size_t)need_out * 8)) {
+        log_error(fn, "input and output overlap (the shift is a gather into a separate buffer)");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    // the kernel indexes rows and columns in 32-bit arithmetic (row + rows per workgroup < 2^31)
+    if (row >= ((int64_t)1 << 30) || col >= ((int64_t)1 << 30) || batch * row >= ((int64_t)1 << 40)) {
+        log_error(fn, "slice extents of 2^30 rows or columns and more are not supported");
+        return ACL_ERROR_API_NOT_SUPPORT;
+    }
+    if (!asdfft::device_available()) {
+        log_error(fn, "no HIP device: this library has no host execution path");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    asdfft::ShiftArgs a{};
+    a.in = static_cast<const float2*>(in);
+    a.out = static_cast<float2*>(out);
+    a.row = row;
+    a.col = col;
+    a.batch = batch;
+    a.dist = dist;
+    a.shift_r = inverse ? row / 2 : (row + 1) / 2;
+    a.shift_c = inverse ? col / 2 : (col + 1) / 2;
+    a.scale = (float)scale;
+    a.apply_scale = inverse && scale != 1.0;
+    a.rows_per_block = (int)std::max<int64_t>(1, std::min<int64_t>(4096 / col, (int64_t)1 << 20));
+    const hipError_t e = asdfft::launch_shift2d(a, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        log_error(fn, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    return ACL_SUCCESS;
 }
 
 }  // namespace
@@ -600,16 +702,18 @@ AspbStatus asdFftExecC2CSeparated(asdFftHandle, const aclTensor*, const aclTenso
     return ACL_ERROR_API_NOT_SUPPORT;
 }
 
-AspbStatus asdFftShift(int64_t, int64_t, int64_t, int64_t, const aclTensor*, aclTensor*, void*, void*)
+AspbStatus asdFftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, const aclTensor* input,
+                       aclTensor* output, void* stream, void* workspace)  // core/fft/fftshift.cpp:101-138
 {
-    log_error("asdFftShift", "fftshift is not part of this engine");
-    return ACL_ERROR_API_NOT_SUPPORT;
+    (void)workspace;
+    return run_shift("asdFftShift", row, col, batch, dist, 1.0, false, input, output, stream);
 }
 
-AspbStatus asdIfftShift(int64_t, int64_t, int64_t, int64_t, double, const aclTensor*, aclTensor*, void*, void*)
+AspbStatus asdIfftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, double scale, const aclTensor* input,
+                        aclTensor* output, void* stream, void* workspace)  // core/fft/ifftshift.cpp
 {
-    log_error("asdIfftShift", "ifftshift is not part of this engine");
-    return ACL_ERROR_API_NOT_SUPPORT;
+    (void)workspace;
+    return run_shift("asdIfftShift", row, col, batch, dist, scale, true, input, output, stream);
 }
 
 }  // extern "C"

@@ -279,8 +279,9 @@ def test_out_of_scope_entries_report_not_supported(L):
         assert L.asdFftMakePlan1D(h, 64, N.ASCEND_FFT_C2C_SEP, FWD, 8, H) == NOT_SUPPORTED
         assert L.asdFftExecC2CSeparated(h, None, None, None, None) == NOT_SUPPORTED
         assert L.asdFftExecIstft(h, None, None, None) == NOT_SUPPORTED
-    assert L.asdFftShift(4, 4, 1, 0, None, None, None, None) == NOT_SUPPORTED
-    assert L.asdIfftShift(4, 4, 1, 0, 1.0, None, None, None, None) == NOT_SUPPORTED
+    # fftshift / ifftshift are implemented (tests/test_shift_contract.py): null tensors are invalid
+    assert L.asdFftShift(4, 4, 1, 0, None, None, None, None) == INVALID
+    assert L.asdIfftShift(4, 4, 1, 0, 1.0, None, None, None, None) == INVALID
 
 
 def test_acl_tensor_accessors(L):

@@ -180,3 +180,40 @@ def test_planted_mistakes_are_caught(oracle, fixtures):
     x = _ut_input(oracle, case)
     wrong_dir = oracle.c2r(x, 30, inverse=True)
     assert _rel(wrong_dir, ut[case["name"]]) > 0.1
+
+
+# ---- 2D circular shift (asdFftShift / asdIfftShift contract) ----------------------------------
+@pytest.mark.parametrize("row,col,batch,dist", [(6, 8, 3, 0), (7, 9, 2, 0), (1, 16, 4, 0), (16, 1, 2, 0),
+                                                (4, 6, 3, 32), (33, 67, 2, 0)])
+def test_shift_matches_numpy_roll(oracle, row, col, batch, dist):
+    """Pinned on numpy.roll, the definition the reference's gen_data.py goldens use:
+    forward shift by ((row+1)//2, (col+1)//2), inverse by (row//2, col//2)."""
+    rng = np.random.default_rng(row * 100 + col)
+    d = row * col if dist <= 0 else dist
+    flat = (rng.standard_normal(d * batch) + 1j * rng.standard_normal(d * batch)).astype(np.complex64)
+    slices = np.stack([flat[s * d:s * d + row * col].reshape(row, col) for s in range(batch)])
+    fwd = oracle.shift2d(flat, row, col, batch, dist, inverse=False)
+    inv = oracle.shift2d(flat, row, col, batch, dist, inverse=True)
+    assert np.array_equal(fwd, np.roll(slices, ((row + 1) // 2, (col + 1) // 2), axis=(1, 2)))
+    assert np.array_equal(inv, np.roll(slices, (row // 2, col // 2), axis=(1, 2)))
+    # numpy's own fftshift is the inverse shift for odd sizes (floor), the forward one agrees for even
+    assert np.array_equal(inv, np.fft.fftshift(slices, axes=(1, 2)))
+    # round trip: inverse undoes forward for every parity
+    assert np.array_equal(oracle.shift2d(fwd, row, col, batch, 0, inverse=True), slices)
+
+
+def test_shift_scale_is_float32_product(oracle):
+    rng = np.random.default_rng(5)
+    flat = (rng.standard_normal(63 * 2) + 1j * rng.standard_normal(63 * 2)).astype(np.complex64)
+    scale = 1.0 / 63.0
+    got = oracle.shift2d(flat, 7, 9, 2, 0, inverse=True, scale=scale)
+    want = np.roll(flat.reshape(2, 7, 9), (3, 4), axis=(1, 2)) * np.float32(scale)
+    assert want.dtype == np.complex64
+    assert np.array_equal(got, want)
+
+
+def test_shift_planted_mistake_is_caught(oracle):
+    """A shift by floor instead of ceil on odd sizes (the easy mistake) differs."""
+    flat = np.arange(7 * 9, dtype=np.float32).astype(np.complex64)
+    fwd = oracle.shift2d(flat, 7, 9, 1, 0, inverse=False)
+    assert not np.array_equal(fwd, np.roll(flat.reshape(1, 7, 9), (3, 4), axis=(1, 2)))
