@@ -19,6 +19,9 @@
  * Tensors: 1D (batch, N) [HORIZONTAL] or (N, batch) [VERTICAL]; 2D (batch, X, Y);
  * 3D (batch, X, Y, Z); COMPLEX64 is interleaved (re, im) float32; data pointers
  * are HIP device pointers.  Exec is asynchronous on the bound HIP stream.
+ * ASCEND_FFT_C2C_SEP: the C2C transform on a pair of FLOAT planes (real, imaginary),
+ * 1D (HORIZONTAL, or VERTICAL of batch 1) and 3D plans; 2D plans reject it as the
+ * reference does.
  */
 #ifndef ASDSIP_FFT_API_H
 #define ASDSIP_FFT_API_H
@@ -100,7 +103,13 @@ AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor* input, cons
 /* Replaces include/fft_api.h:65 (fft_api.cpp:1752-1788). */
 AspbStatus asdFftExecC2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output);
 
-/* Not part of this engine's path: returns ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:68-69. */
+/* C2C of a plan made with ASCEND_FFT_C2C_SEP (1D, or 3D) on separated real / imaginary
+ * FLOAT planes of one shape, (batch, N) or (batch, X, Y, Z).  Checks in the reference's
+ * order: shapes differ -> ACL_ERROR_INTERNAL_ERROR; a non-FLOAT plane ->
+ * ACL_ERROR_OP_INPUT_NOT_MATCH; plan not initialised or shape not the plan's ->
+ * ACL_ERROR_INVALID_PARAM; null data, or a plan of another type -> ACL_ERROR_INTERNAL_ERROR.
+ * outputReal/outputImag may equal inputReal/inputImag (in place).
+ * Replaces include/fft_api.h:68-69 (fft_api.cpp:1846-1985). */
 AspbStatus asdFftExecC2CSeparated(asdFftHandle handle, const aclTensor* inputReal, const aclTensor* inputImag,
                                   const aclTensor* outputReal, const aclTensor* outputImag);
 

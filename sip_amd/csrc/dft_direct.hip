@@ -16,6 +16,8 @@ __device__ __forceinline__ float2 dft_load(const DftArgs& a, int64_t base, int i
     float2 x;
     if (a.prologue == kProReal) {
         x = make_float2(reinterpret_cast<const float*>(a.in)[base + i], 0.0f);
+    } else if (a.prologue == kProPlanar) {
+        x = make_float2(reinterpret_cast<const float*>(a.in)[base + i], a.in_im[base + i]);
     } else if (a.prologue == kProHerm) {
         const int h = a.n >> 1;
         if (i <= h) {
@@ -64,6 +66,10 @@ __global__ void __launch_bounds__(256) dft_direct_kernel(const DftArgs a)
         }
         if (a.epilogue == kEpiReal) {
             reinterpret_cast<float*>(a.out)[t * a.out_dist + k] = accx;
+        } else if (a.epilogue == kEpiPlanar) {
+            if (a.conj_out) accy = -accy;
+            reinterpret_cast<float*>(a.out)[t * a.out_dist + k] = accx;
+            a.out_im[t * a.out_dist + k] = accy;
         } else {
             if (a.conj_out) accy = -accy;
             reinterpret_cast<float2*>(a.out)[t * a.out_dist + k] = make_float2(accx, accy);

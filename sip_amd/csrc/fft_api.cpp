@@ -103,6 +103,7 @@ AspbStatus finish_plan(FftPlan& p, const char* fn)
     spec.dims = p.sizes;
     spec.batch = p.vertical ? p.vertical_batch : p.batch;
     spec.vertical = p.vertical;
+    spec.planar = (p.type == ASCEND_FFT_C2C_SEP);
     spec.precision = p.precision;
     asdfft::BuiltPlan built;
     std::string why;
@@ -188,6 +189,18 @@ bool match_real(const FftPlan& p, const aclTensor* t)
     return leading_extent_matches(d, i + 1, plan_batch_extent(p));
 }
 
+// matchSeparatedPlan_ (fft_api.cpp:1728-1750): trailing extents are the plan's sizes and
+// the leading ones cover exactly its batch
+bool match_separated(const FftPlan& p, const std::vector<int64_t>& d)
+{
+    if (p.sizes.empty() || d.size() < p.sizes.size()) return false;
+    int64_t i = (int64_t)d.size() - 1;
+    for (size_t k = p.sizes.size(); k-- > 0; i--) {
+        if (d[i] != p.sizes[k]) return false;
+    }
+    return leading_extent_matches(d, i + 1, plan_batch_extent(p));
+}
+
 bool ranges_overlap(const void* a, size_t na, const void* b, size_t nb)
 {
     const char* x = static_cast<const char*>(a);
@@ -195,12 +208,54 @@ bool ranges_overlap(const void* a, size_t na, const void* b, size_t nb)
     return x < y + nb && y < x + na;
 }
 
-// asdFftExecV2 (fft_api.cpp:1446-1511): workspace guard, then the launch list
-// (only_step >= 0 launches that one step, for per-kernel timing).
-AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output, const char* fn, int only_step = -1)
+// The launch list of a built plan.  bufs = {input, output, workspace}; a step whose
+// source (destination) is a pair of float planes takes the real plane from bufs and the
+// imaginary plane from in_im (out_im).  only_step >= 0 launches that one step.
+AspbStatus launch_steps(const asdfft::BuiltPlan& bp, char* const bufs[3], const float* in_im, float* out_im,
+                        hipStream_t stream, const char* fn, int only_step)
+{
+    for (size_t si = 0; si < bp.steps.size(); si++) {
+        if (only_step >= 0 && (size_t)only_step != si) continue;
+        const asdfft::Step& s = bp.steps[si];
+        if ((s.planar_in && in_im == nullptr) || (s.planar_out && out_im == nullptr)) {
+            log_error(fn, "separated-plane step without its imaginary plane");
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+        hipError_t e;
+        if (s.mfma) {
+            asdfft::MfmaFftArgs a = s.margs;
+            a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
+            a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
+            e = asdfft::launch_mfma_fft1024(a, s.mfma_grid, stream);
+        } else if (s.direct) {
+            asdfft::DftArgs a = s.dargs;
+            a.in = bufs[s.src] + s.src_offset;
+            a.out = bufs[s.dst] + s.dst_offset;
+            a.in_im = s.planar_in ? in_im : nullptr;
+            a.out_im = s.planar_out ? out_im : nullptr;
+            e = asdfft::launch_dft_direct(s.shape, a, stream);
+        } else {
+            asdfft::FftArgs a = s.args;
+            a.in = bufs[s.src] + s.src_offset;
+            a.out = bufs[s.dst] + s.dst_offset;
+            a.in_im = s.planar_in ? in_im : nullptr;
+            a.out_im = s.planar_out ? out_im : nullptr;
+            e = asdfft::launch_stockham(s.shape, a, stream);
+        }
+        if (e != hipSuccess) {
+            log_error(fn, std::string("kernel launch failed: ") + hipGetErrorString(e));
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    return ACL_SUCCESS;
+}
+
+// CheckWorkspaceBinding (fft_api.cpp:1015-1032): a plan that needs workspace must have
+// had its size queried and a large enough buffer bound.
+AspbStatus check_workspace(const FftPlan& p, const char* fn)
 {
     const asdfft::BuiltPlan& bp = p.built;
-    if (bp.workspace > 0 && p.workspace == nullptr) {  // CheckWorkspaceBinding, fft_api.cpp:1020-1032
+    if (bp.workspace > 0 && p.workspace == nullptr) {
         log_error(fn, "workspace has not been set");
         return ACL_ERROR_INTERNAL_ERROR;
     }
@@ -212,6 +267,32 @@ AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output,
         log_error(fn, "workspace smaller than the plan needs");
         return ACL_ERROR_INVALID_PARAM;
     }
+    return ACL_SUCCESS;
+}
+
+// Every step's device tables exist (a plan made without a device has none).
+AspbStatus check_device(const asdfft::BuiltPlan& bp, const char* fn)
+{
+    if (!asdfft::device_available()) {
+        log_error(fn, "no HIP device: this library has no host execution path");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    for (const asdfft::Step& s : bp.steps) {
+        if (!s.tables_ready) {
+            log_error(fn, "plan was built without a HIP device; create it again on a device");
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    return ACL_SUCCESS;
+}
+
+// asdFftExecV2 (fft_api.cpp:1446-1511): workspace guard, then the launch list
+// (only_step >= 0 launches that one step, for per-kernel timing).
+AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output, const char* fn, int only_step = -1)
+{
+    const asdfft::BuiltPlan& bp = p.built;
+    const AspbStatus ws = check_workspace(p, fn);
+    if (ws != ACL_SUCCESS) return ws;
     void* in = asdfft::data_address(input);
     void* out = asdfft::data_address(output);
     if (in == nullptr || out == nullptr) {
@@ -229,44 +310,10 @@ AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output,
         log_error(fn, "input and output overlap (only exact in-place C2C is supported)");
         return ACL_ERROR_INVALID_PARAM;
     }
-    if (!asdfft::device_available()) {
-        log_error(fn, "no HIP device: this library has no host execution path");
-        return ACL_ERROR_INTERNAL_ERROR;
-    }
-    for (const asdfft::Step& s : bp.steps) {
-        if (!s.tables_ready) {
-            log_error(fn, "plan was built without a HIP device; create it again on a device");
-            return ACL_ERROR_INTERNAL_ERROR;
-        }
-    }
-    hipStream_t stream = static_cast<hipStream_t>(p.stream);
+    const AspbStatus dev = check_device(bp, fn);
+    if (dev != ACL_SUCCESS) return dev;
     char* bufs[3] = {static_cast<char*>(in), static_cast<char*>(out), static_cast<char*>(p.workspace)};
-    for (size_t si = 0; si < bp.steps.size(); si++) {
-        if (only_step >= 0 && (size_t)only_step != si) continue;
-        const asdfft::Step& s = bp.steps[si];
-        hipError_t e;
-        if (s.mfma) {
-            asdfft::MfmaFftArgs a = s.margs;
-            a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
-            a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
-            e = asdfft::launch_mfma_fft1024(a, s.mfma_grid, stream);
-        } else if (s.direct) {
-            asdfft::DftArgs a = s.dargs;
-            a.in = bufs[s.src] + s.src_offset;
-            a.out = bufs[s.dst] + s.dst_offset;
-            e = asdfft::launch_dft_direct(s.shape, a, stream);
-        } else {
-            asdfft::FftArgs a = s.args;
-            a.in = bufs[s.src] + s.src_offset;
-            a.out = bufs[s.dst] + s.dst_offset;
-            e = asdfft::launch_stockham(s.shape, a, stream);
-        }
-        if (e != hipSuccess) {
-            log_error(fn, std::string("kernel launch failed: ") + hipGetErrorString(e));
-            return ACL_ERROR_INTERNAL_ERROR;
-        }
-    }
-    return ACL_SUCCESS;
+    return launch_steps(bp, bufs, nullptr, nullptr, static_cast<hipStream_t>(p.stream), fn, only_step);
 }
 
 template <typename MatchIn, typename MatchOut>
@@ -481,10 +528,6 @@ AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, asdFftType fft
         log_error(fn, "invalid dimType");
         return ACL_ERROR_INVALID_PARAM;
     }
-    if (fftType == ASCEND_FFT_C2C_SEP) {
-        log_error(fn, "separated real/imag planes are not part of this engine");
-        return ACL_ERROR_API_NOT_SUPPORT;
-    }
     if (dimType == ASCEND_FFT_VERTICAL && batchSize > 1 && fftType != ASCEND_FFT_C2C) {
         log_error(fn, "VERTICAL layout with batch > 1 is supported for C2C only");
         return ACL_ERROR_API_NOT_SUPPORT;
@@ -568,10 +611,6 @@ AspbStatus asdFftMakePlan3D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSi
     if (p.state == PlanState::kInitialized) {
         log_error(fn, "plan is already initialized");
         return ACL_ERROR_INVALID_PARAM;
-    }
-    if (fftType == ASCEND_FFT_C2C_SEP) {
-        log_error(fn, "separated real/imag planes are not part of this engine");
-        return ACL_ERROR_API_NOT_SUPPORT;
     }
     const int64_t imax = std::numeric_limits<int32_t>::max();
     const int64_t b = batchSize;
@@ -696,10 +735,88 @@ AspbStatus asdFftExecIstft(asdFftHandle, const aclTensor*, const aclTensor*, con
     return ACL_ERROR_API_NOT_SUPPORT;
 }
 
-AspbStatus asdFftExecC2CSeparated(asdFftHandle, const aclTensor*, const aclTensor*, const aclTensor*, const aclTensor*)
+AspbStatus asdFftExecC2CSeparated(asdFftHandle handle, const aclTensor* inputReal, const aclTensor* inputImag,
+                                  const aclTensor* outputReal, const aclTensor* outputImag)  // :1912-1985
 {
-    log_error("asdFftExecC2CSeparated", "separated real/imag planes are not part of this engine");
-    return ACL_ERROR_API_NOT_SUPPORT;
+    const char* fn = "asdFftExecC2CSeparated";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const aclTensor* t[4] = {inputReal, inputImag, outputReal, outputImag};
+    const char* names[4] = {"inputReal", "inputImag", "outputReal", "outputImag"};
+    for (int i = 0; i < 4; i++) {
+        if (t[i] == nullptr) {  // aclGetViewShape fails
+            log_error(fn, std::string("no view shape for ") + names[i]);
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    for (int i = 1; i < 4; i++) {  // four equal shapes (rank, then extents)
+        if (t[i]->view != t[0]->view) {
+            log_error(fn, "inputReal, inputImag, outputReal and outputImag must have one shape");
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    for (int i = 0; i < 4; i++) {
+        if (t[i]->dtype != ACL_FLOAT) {
+            log_error(fn, std::string("requires a FLOAT ") + names[i] + " tensor");
+            return ACL_ERROR_OP_INPUT_NOT_MATCH;
+        }
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state != PlanState::kInitialized) {
+        log_error(fn, "plan is not initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (p.ragged) {
+        log_error(fn, "ragged-batch plans run through asdFftExtExecRagged");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!match_separated(p, t[0]->view)) {
+        log_error(fn, "input/output shape does not match plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    // asdFftExecV2Separated (:1846-1910)
+    const AspbStatus ws = check_workspace(p, fn);
+    if (ws != ACL_SUCCESS) return ws;
+    char* data[4];
+    for (int i = 0; i < 4; i++) {
+        data[i] = static_cast<char*>(asdfft::data_address(t[i]));
+        if (data[i] == nullptr) {
+            log_error(fn, std::string(names[i]) + " data is nullptr");
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    if (p.type != ASCEND_FFT_C2C_SEP) {  // only separated plans have plane kernels (:1889-1898)
+        log_error(fn, "the plan does not implement separated execution (make it with ASCEND_FFT_C2C_SEP)");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    for (int i = 0; i < 4; i++) {
+        if (!asdfft::is_dense(t[i])) {
+            log_error(fn, "only dense row-major tensors are supported");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+    }
+    // the planes may be updated in place pairwise (outputReal == inputReal and outputImag ==
+    // inputImag); any other overlap between them is rejected
+    const size_t bytes = (size_t)asdfft::numel(t[0]) * sizeof(float);
+    const int pairs[6][2] = {{0, 1}, {2, 3}, {0, 3}, {1, 2}, {0, 2}, {1, 3}};
+    for (int k = 0; k < 6; k++) {
+        const char* x = data[pairs[k][0]];
+        const char* y = data[pairs[k][1]];
+        const bool in_place = (k >= 4 && x == y);
+        if (!in_place && ranges_overlap(x, bytes, y, bytes)) {
+            log_error(fn, "planes overlap (only outputReal == inputReal and outputImag == inputImag in place)");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+    }
+    const AspbStatus dev = check_device(p.built, fn);
+    if (dev != ACL_SUCCESS) return dev;
+    char* bufs[3] = {data[0], data[2], static_cast<char*>(p.workspace)};
+    return launch_steps(p.built, bufs, reinterpret_cast<const float*>(data[1]), reinterpret_cast<float*>(data[3]),
+                        static_cast<hipStream_t>(p.stream), fn, -1);
 }
 
 AspbStatus asdFftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, const aclTensor* input,

@@ -4,6 +4,8 @@
 // points).  Transform t lives at
 //     base(t) = (t % cols) * inner + (t / cols) * outer      (element units)
 // for input and output separately, element i at base(t) + i * elem_stride.
+// Planar (separated real/imaginary) inputs and outputs use the same element
+// offsets in each of their two float planes.
 // Row batches use cols = 1; column passes of 2D/3D plans use cols = number of
 // adjacent columns, inner = 1.
 #pragma once
@@ -28,6 +30,7 @@ enum Prologue : int {
     kProReal = 1,      // real load, imaginary part 0 (odd-length R2C)
     kProHerm = 2,      // Hermitian half -> full spectrum (odd-length C2R)
     kProC2REven = 3,   // even-length C2R pre-twiddle: Z[k] from X[k], X[M-k]
+    kProPlanar = 4,    // real and imaginary parts from two float planes (in, in_im)
 };
 
 // How the last pass disposes of its outputs.
@@ -38,6 +41,8 @@ enum Epilogue : int {
     kEpiR2CEven = 3,   // even-length R2C post-twiddle through LDS
     kEpiTwiddle = 4,   // multiply by W_big^(column * i), then store (four-step pass 1)
     kEpiTranspose = 5, // store transposed through LDS (four-step pass 2)
+    kEpiPlanar = 6,    // real and imaginary parts to two float planes (out, out_im)
+    kEpiTransposePlanar = 7,  // kEpiTranspose into two float planes
 };
 
 struct PassDesc {
@@ -50,6 +55,8 @@ struct PassDesc {
 struct FftArgs {
     const void* in;
     void* out;
+    const float* in_im;     // imaginary plane of a planar input (kProPlanar), else unused
+    float* out_im;          // imaginary plane of a planar output (kEpiPlanar, kEpiTransposePlanar)
     const float* tw;        // per-pass twiddles, interleaved (re, im)
     const float* tw_aux;    // R2C/C2R half-length twiddles W_N^k, k in [0, n]
     const float* tw_big_lo; // four-step twiddles: W_big^m = hi[m >> big_shift] * lo[m & mask]
@@ -101,11 +108,13 @@ struct RaggedArgs {
 struct DftArgs {
     const void* in;
     void* out;
+    const float* in_im;     // planar input / output planes (kProPlanar, kEpiPlanar)
+    float* out_im;
     const float* roots;     // W_N^m, m in [0, N)
     int n;
     int tpb;
     int conj_in, conj_out;
-    int prologue, epilogue; // kProComplex/kProReal/kProHerm, kEpiComplex/kEpiHalf/kEpiReal
+    int prologue, epilogue; // kProComplex/kProReal/kProHerm/kProPlanar, kEpiComplex/kEpiHalf/kEpiReal/kEpiPlanar
     int64_t count;
     int64_t in_dist, out_dist;
 };

@@ -19,6 +19,12 @@ enum Variant : int {
     kVarColsTwiddle = 6,
     kVarRowsTranspose = 7,
     kVarDirect = 8,  // dft_direct_kernel
+    // separated real/imaginary planes (ASCEND_FFT_C2C_SEP plans)
+    kVarRowsSep = 9,            // rows, planar in and out
+    kVarRowsSepIn = 10,         // rows, planar in, complex out
+    kVarColsSepOut = 11,        // columns, complex in, planar out
+    kVarColsTwiddleSep = 12,    // four-step pass 1 with a planar input
+    kVarRowsTransposeSep = 13,  // four-step pass 2 with a planar output
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a
@@ -58,4 +64,19 @@ hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStrea
 #define ASDFFT_FOR_EACH_INSTANCE(X) \
     X(0, 16) X(1, 16) X(2, 16) X(3, 16) X(4, 16) X(5, 16) X(6, 16) X(7, 16) \
     X(0, 32) X(1, 32) X(2, 32) X(3, 32) X(4, 32) X(5, 32) X(6, 32) X(7, 32) \
-    X(0, 48) X(1, 48) X(2, 48) X(3, 48) X(4, 48) X(5, 48)
+    X(0, 48) X(1, 48) X(2, 48) X(3, 48) X(4, 48) X(5, 48) \
+    X(9, 16) X(10, 16) X(11, 16) X(12, 16) X(13, 16) \
+    X(9, 32) X(10, 32) X(11, 32) X(12, 32) X(13, 32) \
+    X(9, 48) X(10, 48) X(11, 48)
+
+namespace asdfft {
+// True when the run-time kernel of (variant, emax) is compiled in.
+inline bool has_instance(int variant, int emax)
+{
+#define ASDFFT_HAS(V, E) \
+    if (variant == V && emax == E) return true;
+    ASDFFT_FOR_EACH_INSTANCE(ASDFFT_HAS)
+#undef ASDFFT_HAS
+    return false;
+}
+}  // namespace asdfft

@@ -124,13 +124,14 @@ std::vector<int> fitting_radices(int64_t n)
     return r;
 }
 
-bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, Config* c)
+bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int variant, Config* c)
 {
     std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
     bool found = false;
     Config best;
     for (int emax : {16, 32, 48}) {
+        if (!has_instance(variant, emax)) continue;
         const int T = threads_for(n, radices, emax);
         if (T == 0) continue;
         const int maxb = max_block_for(emax);
@@ -160,7 +161,7 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, Config* c
 }
 
 // Column mapping: lanes across adjacent transforms (columns) for coalesced strided access.
-bool cols_config(int64_t n, int64_t count, Config* c)
+bool cols_config(int64_t n, int64_t count, int variant, Config* c)
 {
     std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
@@ -183,6 +184,7 @@ bool cols_config(int64_t n, int64_t count, Config* c)
     bool found = false;
     for (int tpb : {16, 8, 4, 2, 1}) {
         for (int emax : {16, 32, 48}) {
+            if (!has_instance(variant, emax)) continue;
             const int T = threads_for(n, radices, emax);
             if (T == 0 || tpb * T > max_block_for(emax)) continue;
             const int64_t plane = n * (tpb + 1);  // column layout: row pitch tpb + 1 words
@@ -239,8 +241,9 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
                    int want_tpb, Step* st, std::string* why)
 {
     Config c;
-    const bool epi_lds = (variant == kVarR2CEven || variant == kVarRowsTranspose);
-    const bool ok = cols_mapping ? cols_config(n, io.count, &c) : rows_config(n, io.count, epi_lds, want_tpb, &c);
+    const bool epi_lds = (variant == kVarR2CEven || variant == kVarRowsTranspose || variant == kVarRowsTransposeSep);
+    const bool ok = cols_mapping ? cols_config(n, io.count, variant, &c)
+                                 : rows_config(n, io.count, epi_lds, want_tpb, variant, &c);
     if (!ok) {
         *why = "no single-workgroup schedule for length " + std::to_string(n);
         return false;
@@ -340,26 +343,37 @@ bool direct_step(int64_t n, int prologue, int epilogue, int64_t count, int64_t i
 bool smooth_single(int64_t n) { return n > 1 && n <= kMaxSingleKernel && !choose_radices(n).empty(); }
 
 // ---- 1D building blocks -------------------------------------------------------------
-// Contiguous complex rows of length n: count rows, in/out row distance n.
-bool c2c_rows(int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPlan* bp, std::string* why)
+// Contiguous complex rows of length n: count rows, in/out row distance n.  pin / pout:
+// the input / output is a pair of float planes (real, imaginary) instead of complex.
+bool c2c_rows(int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPlan* bp, std::string* why,
+              bool pin = false, bool pout = false)
 {
     Step st;
     st.src = src;
     st.dst = dst;
+    st.planar_in = pin;
+    st.planar_out = pout;
     st.bytes = 16.0 * (double)n * (double)count;
     if (n == 1 || !smooth_single(n)) {
         if (n > kMaxSingleKernel && !choose_radices(n).empty()) {
             *why = "large length handled by four-step";
             return false;
         }
-        if (!direct_step(n, kProComplex, kEpiComplex, count, n, n, inverse, inverse, &st, why)) return false;
+        if (!direct_step(n, pin ? kProPlanar : kProComplex, pout ? kEpiPlanar : kEpiComplex, count, n, n, inverse,
+                         inverse, &st, why))
+            return false;
     } else {
+        if (pout && !pin) {
+            *why = "no row kernel with a complex input and a planar output";
+            return false;
+        }
         StepIo io;
         io.count = count;
         io.cols = 1;
         io.in_outer = n;
         io.out_outer = n;
-        if (!stockham_step(n, kVarRowsC2C, false, io, inverse, inverse, 0, &st, why)) return false;
+        const int variant = pin ? (pout ? kVarRowsSep : kVarRowsSepIn) : kVarRowsC2C;
+        if (!stockham_step(n, variant, false, io, inverse, inverse, 0, &st, why)) return false;
     }
     bp->steps.push_back(std::move(st));
     return true;
@@ -367,8 +381,9 @@ bool c2c_rows(int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPla
 
 // Strided complex transforms of length n along an axis with element stride `stride`;
 // `cols` adjacent transforms per outer group, groups `outer` elements apart.
+// pout: the output is a pair of float planes (real, imaginary).
 bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t stride, bool inverse, int src, int dst,
-              BuiltPlan* bp, std::string* why)
+              BuiltPlan* bp, std::string* why, bool pout = false)
 {
     if (n == 1) {
         if (src == dst) return true;  // identity along this axis
@@ -376,8 +391,10 @@ bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t str
         Step st;
         st.src = src;
         st.dst = dst;
+        st.planar_out = pout;
         st.bytes = 16.0 * (double)count;
-        if (!direct_step(1, kProComplex, kEpiComplex, count, 1, 1, inverse, inverse, &st, why)) return false;
+        if (!direct_step(1, kProComplex, pout ? kEpiPlanar : kEpiComplex, count, 1, 1, inverse, inverse, &st, why))
+            return false;
         bp->steps.push_back(std::move(st));
         return true;
     }
@@ -388,6 +405,7 @@ bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t str
     Step st;
     st.src = src;
     st.dst = dst;
+    st.planar_out = pout;
     st.bytes = 16.0 * (double)n * (double)count;
     StepIo io;
     io.count = count;
@@ -398,15 +416,16 @@ bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t str
     io.out_inner = 1;
     io.out_outer = outer;
     io.out_stride = stride;
-    if (!stockham_step(n, kVarColsC2C, true, io, inverse, inverse, 0, &st, why)) return false;
+    if (!stockham_step(n, pout ? kVarColsSepOut : kVarColsC2C, true, io, inverse, inverse, 0, &st, why)) return false;
     bp->steps.push_back(std::move(st));
     return true;
 }
 
 // Large complex length: n = n1 * n2, pass A = length-n2 columns (stride n1) with the
 // W_n^{n1 k2} twiddle folded into its store, pass B = length-n1 rows stored transposed.
+// planar: pass A reads and pass B writes a pair of float planes (real, imaginary).
 bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size_t work_offset, BuiltPlan* bp,
-                  std::string* why)
+                  std::string* why, bool planar = false)
 {
     int64_t best1 = 0;
     double best_score = -1e30;
@@ -415,8 +434,8 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
         const int64_t n2 = n / n1;
         if (n2 > 4096 || !smooth_single(n1) || !smooth_single(n2)) continue;
         Config ca, cb;
-        if (!cols_config(n2, count * n1, &ca)) continue;
-        if (!rows_config(n1, count * n2, true, 16, &cb)) continue;
+        if (!cols_config(n2, count * n1, planar ? kVarColsTwiddleSep : kVarColsTwiddle, &ca)) continue;
+        if (!rows_config(n1, count * n2, true, 16, planar ? kVarRowsTransposeSep : kVarRowsTranspose, &cb)) continue;
         const double score = 4.0 * std::min(ca.tpb, cb.tpb) - fabs(log2((double)n1) - log2((double)n2));
         if (score > best_score) {
             best_score = score;
@@ -440,6 +459,7 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     a.src = src;
     a.dst = kBufWork;
     a.dst_offset = work_offset;
+    a.planar_in = planar;
     a.bytes = 16.0 * (double)n * (double)count;
     StepIo ia;
     ia.count = count * n1;
@@ -450,7 +470,7 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     ia.out_inner = 1;
     ia.out_outer = n;
     ia.out_stride = n1;
-    if (!stockham_step(n2, kVarColsTwiddle, true, ia, inverse, 0, 0, &a, why)) return false;
+    if (!stockham_step(n2, planar ? kVarColsTwiddleSep : kVarColsTwiddle, true, ia, inverse, 0, 0, &a, why)) return false;
     a.args.tw_big_lo = lo ? lo->dev : nullptr;
     a.args.tw_big_hi = hi ? hi->dev : nullptr;
     a.args.big_shift = shift;
@@ -463,6 +483,7 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     b.src = kBufWork;
     b.src_offset = work_offset;
     b.dst = dst;
+    b.planar_out = planar;
     b.bytes = 16.0 * (double)n * (double)count;
     StepIo ib;
     ib.count = count * n2;
@@ -473,7 +494,8 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     ib.out_inner = 1;
     ib.out_outer = n;
     ib.out_stride = n2;
-    if (!stockham_step(n1, kVarRowsTranspose, false, ib, 0, inverse, 16, &b, why)) return false;
+    if (!stockham_step(n1, planar ? kVarRowsTransposeSep : kVarRowsTranspose, false, ib, 0, inverse, 16, &b, why))
+        return false;
     b.describe += " (four-step B)";
     bp->steps.push_back(std::move(a));
     bp->steps.push_back(std::move(b));
@@ -651,8 +673,20 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
     const size_t nd = spec.dims.size();
     if (nd < 1 || nd > 3) return "unsupported rank";
 
+    if (spec.planar && spec.kind != Kind::C2C) return "separated real/imaginary planes are C2C only";
     if (nd == 1) {
         const int64_t n = spec.dims[0];
+        if (spec.planar) {
+            // (batch, n) real plane and imaginary plane; a VERTICAL plan of batch 1 is the same
+            if (spec.vertical && B > 1) return "separated planes with a VERTICAL batch are not supported";
+            if (n > kMaxSingleKernel && !choose_radices(n).empty()) {
+                out->workspace = (size_t)(B * n * 8);
+                if (!c2c_fourstep(n, B, inv, kBufIn, kBufOut, 0, out, &why, true)) return why;
+                return "";
+            }
+            if (!c2c_rows(n, B, inv, kBufIn, kBufOut, out, &why, true, true)) return why;
+            return "";
+        }
         if (spec.vertical) {
             // tensor (n, batch): transform along axis 0, `batch` adjacent columns
             if (spec.kind != Kind::C2C) return "VERTICAL layout is supported for C2C only";
@@ -688,7 +722,7 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
     const int64_t rows = B * lead;
 
     // column passes over the complex array [B][dims[0..nd-2]][W], axis d (d < nd-1)
-    auto col_pass = [&](size_t d, int src, int dst) -> bool {
+    auto col_pass = [&](size_t d, int src, int dst, bool pout = false) -> bool {
         const int64_t len = spec.dims[d];
         int64_t inner = W;  // elements between consecutive points along axis d
         for (size_t e = d + 1; e + 1 < nd; e++) inner *= spec.dims[e];
@@ -697,8 +731,29 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
         if (len == 1) {
             if (src == dst) return true;
         }
-        return c2c_cols(len, count, inner, outer, inner, inv, src, dst, out, &why);
+        return c2c_cols(len, count, inner, outer, inner, inv, src, dst, out, &why, pout);
     };
+
+    if (spec.planar) {
+        // rows read the planes into the complex workspace; the last non-trivial column
+        // pass writes the planes (a plan whose leading axes all have length 1 is rows only)
+        if (L > kMaxSingleKernel) return "last-axis length above " + std::to_string(kMaxSingleKernel);
+        std::vector<size_t> axes;
+        for (size_t d = nd - 1; d-- > 0;) {
+            if (spec.dims[d] > 1) axes.push_back(d);
+        }
+        if (axes.empty()) {
+            if (!c2c_rows(L, rows, inv, kBufIn, kBufOut, out, &why, true, true)) return why;
+            return "";
+        }
+        out->workspace = (size_t)(rows * L * 8);
+        if (!c2c_rows(L, rows, inv, kBufIn, kBufWork, out, &why, true, false)) return why;
+        for (size_t i = 0; i < axes.size(); i++) {
+            const bool last = (i + 1 == axes.size());
+            if (!col_pass(axes[i], kBufWork, last ? kBufOut : kBufWork, last)) return why;
+        }
+        return "";
+    }
 
     if (spec.kind == Kind::C2C) {
         if (L > kMaxSingleKernel) return "last-axis length above " + std::to_string(kMaxSingleKernel);
@@ -739,7 +794,8 @@ std::string describe_plan(const BuiltPlan& plan)
     os << "plan: " << plan.steps.size() << " step(s), workspace=" << plan.workspace << " B\n";
     const char* names[] = {"in", "out", "work"};
     for (const Step& s : plan.steps) {
-        os << "  [" << names[s.src] << "->" << names[s.dst] << "] " << s.describe << "\n";
+        os << "  [" << names[s.src] << (s.planar_in ? "(re,im)" : "") << "->" << names[s.dst]
+           << (s.planar_out ? "(re,im)" : "") << "] " << s.describe << "\n";
     }
     return os.str();
 }

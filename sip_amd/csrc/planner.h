@@ -72,6 +72,8 @@ struct Step {
     DftArgs dargs{};
     int src = kBufIn;
     int dst = kBufOut;
+    bool planar_in = false;   // src is a pair of float planes (real, imaginary)
+    bool planar_out = false;  // dst is a pair of float planes
     size_t src_offset = 0;  // bytes into the buffer
     size_t dst_offset = 0;
     std::vector<TablePtr> tables;
@@ -87,6 +89,7 @@ struct PlanSpec {
     std::vector<int64_t> dims;  // FFT sizes; the last one is the contiguous axis
     int64_t batch = 1;
     bool vertical = false;      // 1D VERTICAL layout: tensor (N, batch)
+    bool planar = false;        // C2C on separated real / imaginary float planes (C2C_SEP)
     Precision precision = Precision::kFp32;
 };
 

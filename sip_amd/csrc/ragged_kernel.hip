@@ -37,6 +37,8 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& r, const FftArgs& 
     Io io;
     io.in = r.in_base;
     io.out = r.out_base;
+    io.in_im = nullptr;  // ragged classes are interleaved complex or real
+    io.out_im = nullptr;
     io.col = 0;
     io.cin = __builtin_amdgcn_readfirstlane(a.conj_in) ? 0x80000000u : 0u;
     io.cout = __builtin_amdgcn_readfirstlane(a.conj_out) ? 0x80000000u : 0u;

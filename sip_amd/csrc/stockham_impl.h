@@ -282,6 +282,8 @@ __device__ __forceinline__ float2 conj_if(float2 x, uint32_t mask)
 struct Io {
     const void* in;   // first element of this transform's input
     void* out;        // first element of this transform's output
+    const float* in_im;  // planar input: first element of the imaginary plane
+    float* out_im;       // planar output: first element of the imaginary plane
     int64_t col;      // column index (four-step twiddle)
     uint32_t cin;     // sign mask applied to inputs (conj_in)
     uint32_t cout;    // sign mask applied to outputs (conj_out)
@@ -305,6 +307,8 @@ __device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int
         return in2[i * s];  // conjugation (conj_in) is applied by the caller
     } else if constexpr (PRO == kProReal) {
         return make_float2(reinterpret_cast<const float*>(io.in)[i * s], 0.0f);
+    } else if constexpr (PRO == kProPlanar) {
+        return make_float2(reinterpret_cast<const float*>(io.in)[i * s], io.in_im[i * s]);  // conj_in by the caller
     } else if constexpr (PRO == kProHerm) {
         const int h = n >> 1;
         float2 x;
@@ -344,6 +348,9 @@ __device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int
         if (2 * i <= n) out2[i * s] = x;
     } else if constexpr (EPI == kEpiReal) {
         reinterpret_cast<float*>(io.out)[i * s] = x.x;
+    } else if constexpr (EPI == kEpiPlanar) {
+        reinterpret_cast<float*>(io.out)[i * s] = x.x;  // conj_out applied by the caller
+        io.out_im[i * s] = x.y;
     } else if constexpr (EPI == kEpiTwiddle) {
         const uint32_t m = (uint32_t)io.col * (uint32_t)i;  // col * i < big length <= 2^31
         const uint32_t mask = (1u << a.big_shift) - 1u;
@@ -376,7 +383,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
     constexpr int B = EMAX / R;
     static_assert(B >= 1, "radix larger than the per-thread register budget");
-    constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose);
+    constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar);
     if constexpr (RT) {
         asm volatile("" : "+s"(g.n), "+s"(g.T));
         asm volatile("" : "+v"(tid));
@@ -406,7 +413,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
             }
         }
-        if constexpr (PRO == kProComplex) {
+        if constexpr (PRO == kProComplex || PRO == kProPlanar) {
             if (io.cin) {  // uniform: one scalar branch instead of a XOR per point
 #pragma unroll
                 for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
@@ -472,7 +479,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
             __syncthreads();
         }
     } else {
-        if constexpr (EPI == kEpiComplex || EPI == kEpiHalf) {
+        if constexpr (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiPlanar) {
             if (io.cout) {  // uniform branch
 #pragma unroll
                 for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
@@ -534,6 +541,8 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
     live = t < a.count;
     io.in = a.in;
     io.out = a.out;
+    io.in_im = a.in_im;
+    io.out_im = a.out_im;
     io.col = 0;
     io.cin = a.conj_in ? 0x80000000u : 0u;
     io.cout = a.conj_out ? 0x80000000u : 0u;
@@ -546,11 +555,17 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
         const int64_t ob = io.col * a.out_inner + o * a.out_outer;
         if constexpr (PRO == kProReal) {
             io.in = reinterpret_cast<const float*>(a.in) + ib;
+        } else if constexpr (PRO == kProPlanar) {
+            io.in = reinterpret_cast<const float*>(a.in) + ib;
+            io.in_im = a.in_im + ib;
         } else {
             io.in = reinterpret_cast<const float2*>(a.in) + ib;
         }
         if constexpr (EPI == kEpiReal) {
             io.out = reinterpret_cast<float*>(a.out) + ob;
+        } else if constexpr (EPI == kEpiPlanar) {
+            io.out = reinterpret_cast<float*>(a.out) + ob;
+            io.out_im = a.out_im + ob;
         } else {
             io.out = reinterpret_cast<float2*>(a.out) + ob;
         }
@@ -587,7 +602,7 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                 if (m - k != k) out2[m - k] = conj_if(x1, io.cout);
             }
         }
-    } else if constexpr (EPI == kEpiTranspose) {
+    } else if constexpr (EPI == kEpiTranspose || EPI == kEpiTransposePlanar) {
         // LDS holds tpb finished transforms; adjacent lanes take adjacent transforms so the
         // strided destination (transform t -> column) is written in contiguous runs.
         const int lane = threadIdx.x;
@@ -596,11 +611,23 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
         const int64_t t2 = blk * g.tpb + tt2;
         if (t2 < a.count) {
             const int64_t c2 = t2 % a.cols;
-            float2* ob = reinterpret_cast<float2*>(a.out) + c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
+            const int64_t o = c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
             const int s = (int)a.out_stride;
-            for (int i = lane / g.tpb; i < g.n; i += step) {
-                const int at = lds_at<MAP>(g, tt2, i);
-                ob[i * s] = conj_if(make_float2(lre[at], lim[at]), io.cout);
+            if constexpr (EPI == kEpiTranspose) {
+                float2* ob = reinterpret_cast<float2*>(a.out) + o;
+                for (int i = lane / g.tpb; i < g.n; i += step) {
+                    const int at = lds_at<MAP>(g, tt2, i);
+                    ob[i * s] = conj_if(make_float2(lre[at], lim[at]), io.cout);
+                }
+            } else {
+                float* obr = reinterpret_cast<float*>(a.out) + o;
+                float* obi = a.out_im + o;
+                const float sgn = io.cout ? -1.0f : 1.0f;
+                for (int i = lane / g.tpb; i < g.n; i += step) {
+                    const int at = lds_at<MAP>(g, tt2, i);
+                    obr[i * s] = lre[at];
+                    obi[i * s] = sgn * lim[at];
+                }
             }
         }
     }
@@ -716,6 +743,11 @@ template <> struct VariantTraits<kVarR2COdd> { static constexpr int map = kMapRo
 template <> struct VariantTraits<kVarC2ROdd> { static constexpr int map = kMapRows, pro = kProHerm, epi = kEpiReal; };
 template <> struct VariantTraits<kVarColsTwiddle> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiTwiddle; };
 template <> struct VariantTraits<kVarRowsTranspose> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiTranspose; };
+template <> struct VariantTraits<kVarRowsSep> { static constexpr int map = kMapRows, pro = kProPlanar, epi = kEpiPlanar; };
+template <> struct VariantTraits<kVarRowsSepIn> { static constexpr int map = kMapRows, pro = kProPlanar, epi = kEpiComplex; };
+template <> struct VariantTraits<kVarColsSepOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiPlanar; };
+template <> struct VariantTraits<kVarColsTwiddleSep> { static constexpr int map = kMapCols, pro = kProPlanar, epi = kEpiTwiddle; };
+template <> struct VariantTraits<kVarRowsTransposeSep> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiTransposePlanar; };
 
 template <typename K>
 hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipStream_t stream)

@@ -20,11 +20,13 @@ import threading
 
 from . import _native as N
 
-_TYPE = {"c2c": N.ASCEND_FFT_C2C, "r2c": N.ASCEND_FFT_R2C, "c2r": N.ASCEND_FFT_C2R}
+_TYPE = {"c2c": N.ASCEND_FFT_C2C, "r2c": N.ASCEND_FFT_R2C, "c2r": N.ASCEND_FFT_C2R,
+         "c2c_sep": N.ASCEND_FFT_C2C_SEP}
 
 
 class Plan:
-    """One plan handle.  dims: FFT sizes (1-3 of them); kind: 'c2c' | 'r2c' | 'c2r'."""
+    """One plan handle.  dims: FFT sizes (1-3 of them); kind: 'c2c' | 'r2c' | 'c2r' | 'c2c_sep'
+    (C2C on separated real / imaginary float32 planes, run by execute_separated)."""
 
     def __init__(self, kind: str, dims, batch: int, inverse: bool = False, vertical: bool = False,
                  precision: str = "fp32"):
@@ -110,6 +112,8 @@ class Plan:
     def execute(self, x, out=None):
         """Runs the plan on torch device tensor x (complex64 or float32); returns out."""
         import torch
+        if self.kind == "c2c_sep":
+            raise TypeError("a c2c_sep plan runs through execute_separated")
         if out is None:
             dt = torch.float32 if self.kind == "c2r" else torch.complex64
             out = torch.empty(self.out_shape(), dtype=dt, device=x.device)
@@ -125,6 +129,25 @@ class Plan:
             xin.close()
             xout.close()
         return out
+
+    def execute_separated(self, re, im, out_re=None, out_im=None):
+        """asdFftExecC2CSeparated on float32 device planes re, im (one shape); returns
+        (out_re, out_im).  out_re / out_im may be re / im (in place)."""
+        import torch
+        if out_re is None:
+            out_re = torch.empty_like(re)
+        if out_im is None:
+            out_im = torch.empty_like(im)
+        self.bind_workspace(re.device)
+        self.bind_stream(torch.cuda.current_stream(re.device).cuda_stream)
+        ts = [N.AclTensor(t.shape, N.ACL_FLOAT if t.dtype == torch.float32 else N.ACL_COMPLEX64, t.data_ptr())
+              for t in (re, im, out_re, out_im)]
+        try:
+            N.check("asdFftExecC2CSeparated", self.L.asdFftExecC2CSeparated(self.handle, *[t.ptr for t in ts]))
+        finally:
+            for t in ts:
+                t.close()
+        return out_re, out_im
 
     def synchronize(self):
         N.check("asdFftSynchronize", self.L.asdFftSynchronize(self.handle))
@@ -375,3 +398,20 @@ def asd_fft_c2r(x, is_forward: bool = True):
     batch, dims = _split(x.shape, "c2r")
     dims = dims[:-1] + (2 * (dims[-1] - 1),)
     return _cache.get("c2r", dims, batch, not is_forward).execute(x)
+
+
+def asd_fft_c2c_sep(in_real, in_imag, is_forward: bool = True):
+    """C2C on separated float32 planes: (batch, N) -> 1D, (batch, X, Y, Z) -> 3D; returns
+    (out_real, out_imag).  Mirrors sip_pta/csrc/fft/asd_fft_c2c_sep.cpp:22-65 (float32
+    check, rank 2 or 4, equal shapes, batch from the leading dimension, HORIZONTAL)."""
+    import torch
+    if in_real.dtype != torch.float32 or in_imag.dtype != torch.float32:
+        raise TypeError(f"asd_fft_c2c_sep: input real/imag must be float32, got {in_real.dtype}/{in_imag.dtype}")
+    in_real = in_real.contiguous()
+    in_imag = in_imag.contiguous()
+    if in_real.dim() not in (2, 4):
+        raise ValueError(f"Separated FFT support 2 or 4 dimensions (including Batch), but got {in_real.dim()}")
+    if in_real.shape != in_imag.shape:
+        raise ValueError("Real and Imaginary tensors must have the same shape")
+    batch, dims = int(in_real.shape[0]), tuple(int(s) for s in in_real.shape[1:])
+    return _cache.get("c2c_sep", dims, batch, not is_forward).execute_separated(in_real, in_imag)

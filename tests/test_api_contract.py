@@ -276,9 +276,8 @@ def test_unsupported_lengths_rejected(L):
 
 def test_out_of_scope_entries_report_not_supported(L):
     with Handle(L) as h:
-        assert L.asdFftMakePlan1D(h, 64, N.ASCEND_FFT_C2C_SEP, FWD, 8, H) == NOT_SUPPORTED
-        assert L.asdFftExecC2CSeparated(h, None, None, None, None) == NOT_SUPPORTED
         assert L.asdFftExecIstft(h, None, None, None) == NOT_SUPPORTED
+    # separated planes are implemented (tests/test_separated_contract.py)
     # fftshift / ifftshift are implemented (tests/test_shift_contract.py): null tensors are invalid
     assert L.asdFftShift(4, 4, 1, 0, None, None, None, None) == INVALID
     assert L.asdIfftShift(4, 4, 1, 0, 1.0, None, None, None, None) == INVALID
