@@ -181,8 +181,12 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
             return true;
         }
     }
+    // 16 adjacent complex columns are 128-byte runs; a float plane needs 32 for the same
+    // (a half-line read by two workgroups on different XCDs is fetched twice)
+    const int first_tpb = (variant == kVarColsTwiddleSep) ? 32 : 16;
     bool found = false;
-    for (int tpb : {16, 8, 4, 2, 1}) {
+    for (int tpb : {32, 16, 8, 4, 2, 1}) {
+        if (tpb > first_tpb) continue;
         for (int emax : {16, 32, 48}) {
             if (!has_instance(variant, emax)) continue;
             const int T = threads_for(n, radices, emax);
@@ -435,7 +439,9 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
         if (n2 > 4096 || !smooth_single(n1) || !smooth_single(n2)) continue;
         Config ca, cb;
         if (!cols_config(n2, count * n1, planar ? kVarColsTwiddleSep : kVarColsTwiddle, &ca)) continue;
-        if (!rows_config(n1, count * n2, true, 16, planar ? kVarRowsTransposeSep : kVarRowsTranspose, &cb)) continue;
+        if (!rows_config(n1, count * n2, true, planar ? 32 : 16, planar ? kVarRowsTransposeSep : kVarRowsTranspose,
+                         &cb))
+            continue;
         const double score = 4.0 * std::min(ca.tpb, cb.tpb) - fabs(log2((double)n1) - log2((double)n2));
         if (score > best_score) {
             best_score = score;
@@ -494,7 +500,10 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     ib.out_inner = 1;
     ib.out_outer = n;
     ib.out_stride = n2;
-    if (!stockham_step(n1, planar ? kVarRowsTransposeSep : kVarRowsTranspose, false, ib, 0, inverse, 16, &b, why))
+    // transposed stores: 16 transforms per workgroup write 128-byte runs of complex
+    // values, 32 those of a float plane
+    if (!stockham_step(n1, planar ? kVarRowsTransposeSep : kVarRowsTranspose, false, ib, 0, inverse, planar ? 32 : 16,
+                       &b, why))
         return false;
     b.describe += " (four-step B)";
     bp->steps.push_back(std::move(a));
