@@ -467,6 +467,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision form")
     ap.add_argument("--no-separated", action="store_true", help="skip the separated-planes form")
+    ap.add_argument("--separated-only", action="store_true", help="run only the separated-planes form")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
@@ -487,6 +488,9 @@ def main():
         return
     if a.serve_only:
         print(json.dumps({"forms": {"serving": bench_serve(torch, dev)}}))
+        return
+    if a.separated_only:
+        print(json.dumps({"forms": {"separated": bench_separated(torch, dev, stream, max(5, a.kernel_reps))}}))
         return
     L = N.lib()
     entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
