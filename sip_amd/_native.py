@@ -11,6 +11,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libasdfft.so")
+TORCH_LIB_PATH = os.path.join(HERE, "lib", "libasdfft_torch.so")
 
 # asdFftType / asdFftDirection / asdFft1dDimType (include/fft_api.h)
 ASCEND_FFT_C2C = 0x10
@@ -174,6 +175,23 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+_torch_ops = None
+
+
+def torch_ops():
+    """Load libasdfft_torch.so (the torch_sip operators) once and return torch.ops.torch_sip;
+    raises NativeLibraryMissing if it has not been built."""
+    global _torch_ops
+    if _torch_ops is None:
+        import torch
+        if not os.path.exists(TORCH_LIB_PATH):
+            raise NativeLibraryMissing(f"{TORCH_LIB_PATH} not built; run `python -m sip_amd.build`")
+        lib()  # the C ABI library first (the operators link against it)
+        torch.ops.load_library(TORCH_LIB_PATH)
+        _torch_ops = torch.ops.torch_sip
+    return _torch_ops
 
 
 def check(fn: str, status: int) -> None:

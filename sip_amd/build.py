@@ -22,6 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libasdfft.so")
+TORCH_LIB = os.path.join(LIBDIR, "libasdfft_torch.so")
 ARCH = os.environ.get("ASDFFT_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
@@ -117,6 +118,25 @@ def _run(cmd):
     return out
 
 
+def build_torch_ops(force: bool = False) -> str:
+    """libasdfft_torch.so: the torch_sip operators (torch_ops.cpp) over libasdfft.so."""
+    import torch
+    tdir = os.path.dirname(torch.__file__)
+    src = os.path.join(CSRC, "torch_ops.cpp")
+    deps = [src, LIB] + [os.path.join(INCLUDE, h) for h in os.listdir(INCLUDE) if h.endswith(".h")]
+    if not force and not _newer(TORCH_LIB, deps):
+        return TORCH_LIB
+    abi = int(torch.compiled_with_cxx11_abi())
+    tmp = TORCH_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DUSE_ROCM",
+          "-D__HIP_PLATFORM_AMD__", f"-I{tdir}/include", f"-I{tdir}/include/torch/csrc/api/include",
+          f"-I{ROCM}/include", f"-I{INCLUDE}", src, "-o", tmp, f"-L{tdir}/lib", "-lc10", "-lc10_hip", "-ltorch",
+          "-ltorch_cpu", "-ltorch_hip", f"-L{LIBDIR}", "-lasdfft", f"-L{ROCM}/lib", "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"])
+    os.replace(tmp, TORCH_LIB)
+    return TORCH_LIB
+
+
 def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -169,6 +189,7 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
         _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
              [f"-L{ROCM}/lib", "-lamdhip64", "-Wl,--no-undefined"])
         os.replace(tmp, LIB)
+    build_torch_ops(force)
     return LIB
 
 

@@ -4,19 +4,17 @@
 destroy, the same sequence as the reference's tests, e.g.
 tests/ut/unittest/fft/1d/c2c/test_fft_c2c_1d.cpp:103-136).
 
-`asd_fft_c2c / asd_fft_r2c / asd_fft_c2r` mirror the reference's torch plugin
-operators (sip_pta/torch_sip/__init__.py:427-447, sip_pta/csrc/fft/asd_fft_c2c.cpp:20-55,
-asd_fft_r2c.cpp, asd_fft_c2r.cpp): a rank-2/3/4 tensor becomes a 1D/2D/3D plan
-whose batch is the leading dimension; plans are kept in a small LRU cache
-(sip_pta/csrc/FFTMixCache.cpp:18-40, capacity 10) and the workspace comes from
-torch's caching allocator.  PyTorch is only used for device memory and streams;
-every transform runs in the HIP kernels of libasdfft.so.
+`asd_fft_c2c / asd_fft_c2c_sep / asd_fft_r2c / asd_fft_c2r / fft_shift / ifft_shift`
+are the reference's torch plugin functions (sip_pta/torch_sip/__init__.py:427-540):
+each calls the torch operator torch.ops.torch_sip.* of libasdfft_torch.so
+(sip_amd/csrc/torch_ops.cpp), which keeps plans in a 10-entry LRU, takes the
+workspace from torch's caching allocator and launches on the current stream.
+PyTorch is only used for device memory and streams; every transform runs in the
+HIP kernels of libasdfft.so.
 """
 from __future__ import annotations
 
-import collections
 import ctypes
-import threading
 
 from . import _native as N
 
@@ -105,7 +103,10 @@ class Plan:
             N.check("asdFftSetWorkspace", self.L.asdFftSetWorkspace(self.handle, self._workspace.data_ptr()))
 
     def bind_stream(self, stream_handle: int):
-        if stream_handle and stream_handle != self._stream:
+        # torch's default stream is the null stream, refused by asdFftSetStream; hipStreamLegacy
+        # ((hipStream_t)1) names that same stream
+        stream_handle = stream_handle or 1
+        if stream_handle != self._stream:
             N.check("asdFftSetStream", self.L.asdFftSetStream(self.handle, ctypes.c_void_p(stream_handle)))
             self._stream = stream_handle
 
@@ -336,82 +337,52 @@ class Server:
             pass
 
 
-class PlanCache:
-    """LRU of plans keyed by (kind, dims, batch, inverse); the reference keeps 10."""
-
-    def __init__(self, capacity: int = 10):
-        self.capacity = capacity
-        self._d = collections.OrderedDict()
-        self._mu = threading.Lock()
-
-    def get(self, kind, dims, batch, inverse):
-        key = (kind, tuple(dims), int(batch), bool(inverse))
-        with self._mu:
-            p = self._d.get(key)
-            if p is not None:
-                self._d.move_to_end(key)
-                return p
-        p = Plan(kind, dims, batch, inverse)
-        with self._mu:
-            self._d[key] = p
-            while len(self._d) > self.capacity:
-                _, old = self._d.popitem(last=False)
-                old.close()
-        return p
-
-
-_cache = PlanCache()
-
-
-def _split(shape, kind):
-    if len(shape) < 2 or len(shape) > 4:
-        raise ValueError(f"input must have 2, 3 or 4 dimensions, got {len(shape)}")
-    return int(shape[0]), tuple(int(s) for s in shape[1:])
+def _ops():
+    return N.torch_ops()
 
 
 def asd_fft_c2c(x, is_forward: bool = True):
     """complex64 (batch, N) / (batch, X, Y) / (batch, X, Y, Z) -> same shape."""
-    import torch
-    if x.dtype != torch.complex64:
-        raise TypeError(f"asd_fft_c2c: input must be complex64, got {x.dtype}")
-    x = x.contiguous()
-    batch, dims = _split(x.shape, "c2c")
-    return _cache.get("c2c", dims, batch, not is_forward).execute(x)
+    return _ops().asd_fft_c2c(x, is_forward)
+
+
+def asd_fft_c2c_sep(in_real, in_imag, is_forward: bool = True):
+    """float32 planes (batch, N) -> 1D or (batch, X, Y, Z) -> 3D; returns (out_real, out_imag)."""
+    return _ops().asd_fft_c2c_sep(in_real, in_imag, is_forward)
 
 
 def asd_fft_r2c(x, is_forward: bool = True):
     """float32 (batch, ..., N) -> complex64 (batch, ..., N/2+1)."""
-    import torch
-    if x.dtype != torch.float32:
-        raise TypeError("asd_fft_r2c: input tensor must be float32")
-    x = x.contiguous()
-    batch, dims = _split(x.shape, "r2c")
-    return _cache.get("r2c", dims, batch, not is_forward).execute(x)
+    return _ops().asd_fft_r2c(x, is_forward)
 
 
 def asd_fft_c2r(x, is_forward: bool = True):
     """complex64 (batch, ..., M) -> float32 (batch, ..., 2(M-1)), as the reference plugin."""
-    import torch
-    if x.dtype != torch.complex64:
-        raise TypeError("asd_fft_c2r: input tensor must be complex64")
-    x = x.contiguous()
-    batch, dims = _split(x.shape, "c2r")
-    dims = dims[:-1] + (2 * (dims[-1] - 1),)
-    return _cache.get("c2r", dims, batch, not is_forward).execute(x)
+    return _ops().asd_fft_c2r(x, is_forward)
 
 
-def asd_fft_c2c_sep(in_real, in_imag, is_forward: bool = True):
-    """C2C on separated float32 planes: (batch, N) -> 1D, (batch, X, Y, Z) -> 3D; returns
-    (out_real, out_imag).  Mirrors sip_pta/csrc/fft/asd_fft_c2c_sep.cpp:22-65 (float32
-    check, rank 2 or 4, equal shapes, batch from the leading dimension, HORIZONTAL)."""
-    import torch
-    if in_real.dtype != torch.float32 or in_imag.dtype != torch.float32:
-        raise TypeError(f"asd_fft_c2c_sep: input real/imag must be float32, got {in_real.dtype}/{in_imag.dtype}")
-    in_real = in_real.contiguous()
-    in_imag = in_imag.contiguous()
-    if in_real.dim() not in (2, 4):
-        raise ValueError(f"Separated FFT support 2 or 4 dimensions (including Batch), but got {in_real.dim()}")
-    if in_real.shape != in_imag.shape:
-        raise ValueError("Real and Imaginary tensors must have the same shape")
-    batch, dims = int(in_real.shape[0]), tuple(int(s) for s in in_real.shape[1:])
-    return _cache.get("c2c_sep", dims, batch, not is_forward).execute_separated(in_real, in_imag)
+def fft_shift(x, row: int | None = None, col: int | None = None, batch: int | None = None, dist: int = 0):
+    """Zero frequency to the centre of each (row, col) slice: out[s][(r + ceil(row/2)) % row]
+    [(c + ceil(col/2)) % col] = in[s][r][c].  A 3D (batch, row, col) tensor may omit
+    row/col/batch; a flat layout names them (and dist, the slice distance, <= 0: contiguous).
+    Mirrors sip_pta/torch_sip/__init__.py fft_shift."""
+    if row is None or col is None or batch is None:
+        if x.dim() != 3:
+            raise ValueError(f"fft_shift: auto-infer requires a 3D [batch, row, col] tensor, got {x.dim()}D shape "
+                             f"{tuple(x.shape)}. Explicitly pass row/col/batch for flattened input.")
+        return _ops().asd_fft_shift_auto(x)
+    return _ops().asd_fft_shift(x, row, col, batch, dist)
+
+
+def ifft_shift(x, row: int | None = None, col: int | None = None, batch: int | None = None, dist: int = 0,
+               scale: float = 1.0):
+    """Inverse of fft_shift (floor instead of ceil) times float32(scale); same layouts.
+    Mirrors sip_pta/torch_sip/__init__.py ifft_shift."""
+    if row is None or col is None or batch is None:
+        if x.dim() != 3:
+            raise ValueError(f"ifft_shift: auto-infer requires a 3D [batch, row, col] tensor, got {x.dim()}D shape "
+                             f"{tuple(x.shape)}. Explicitly pass row/col/batch for flattened input.")
+        if dist != 0 or scale != 1.0:
+            return _ops().asd_ifft_shift(x, x.shape[1], x.shape[2], x.shape[0], dist, scale)
+        return _ops().asd_ifft_shift_auto(x)
+    return _ops().asd_ifft_shift(x, row, col, batch, dist, scale)

@@ -202,13 +202,16 @@ def test_describe_marks_plane_steps(L):
 
 
 def test_torch_binding_checks():
+    # the operator's own checks (sip_pta/test/fft/c2c_sep.py:57-81 expects RuntimeError naming float32)
     torch = pytest.importorskip("torch")
     from sip_amd import asd_fft_c2c_sep
-    with pytest.raises(TypeError, match="float32"):
+    with pytest.raises(RuntimeError, match="float32"):
         asd_fft_c2c_sep(torch.zeros(4, 8, dtype=torch.complex64), torch.zeros(4, 8, dtype=torch.complex64))
-    with pytest.raises(TypeError, match="float32"):
+    with pytest.raises(RuntimeError, match="float32"):
         asd_fft_c2c_sep(torch.zeros(4, 8, dtype=torch.int32), torch.zeros(4, 8, dtype=torch.int32))
-    with pytest.raises(ValueError, match="2 or 4"):
+    with pytest.raises(RuntimeError, match="2 or 4"):
         asd_fft_c2c_sep(torch.zeros(4, 8, 8), torch.zeros(4, 8, 8))
-    with pytest.raises(ValueError, match="same shape"):
+    with pytest.raises(RuntimeError, match="same shape"):
         asd_fft_c2c_sep(torch.zeros(4, 8), torch.zeros(4, 16))
+    with pytest.raises(RuntimeError, match="HIP device"):
+        asd_fft_c2c_sep(torch.zeros(4, 8), torch.zeros(4, 8))
