@@ -94,7 +94,12 @@ AspbStatus asdFftSetStream(asdFftHandle handle, void* stream);
 AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, asdFftType fftType, asdFftDirection direction,
                             int64_t batchSize, asdFft1dDimType dimType ASDFFT_DEFAULT(ASCEND_FFT_HORIZONTAL));
 
-/* Not part of this engine's path: returns ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:59-62. */
+/* ISTFT plan for a complex64 (channel, nFft, nFrames) spectrum: center = true, normalized = false,
+ * onesided = false, lengthOpt = 0, returnComplex = true (other values are replaced with a warning,
+ * as the reference does).  Requires nFrames > 1, 0 < hopLengthOpt <= winLengthOpt == nFft,
+ * nFft < 1500, hopLengthOpt < 1500 (else ACL_ERROR_INVALID_PARAM); an initialised handle ->
+ * ACL_ERROR_INTERNAL_ERROR.  The workspace holds the inverse-transformed frames.
+ * Replaces include/fft_api.h:59-62 (core/fft/istft.cpp:343-433, 525-570). */
 AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor* input, const int64_t nFft,
                                const int64_t hopLengthOpt, const int64_t winLengthOpt, const bool center,
                                const bool normalized, const bool onesidedOpt, int64_t lengthOpt,
@@ -119,7 +124,12 @@ AspbStatus asdFftExecC2R(asdFftHandle handle, const aclTensor* input, const aclT
 /* Replaces include/fft_api.h:75 (fft_api.cpp:1818-1844). */
 AspbStatus asdFftExecR2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output);
 
-/* Not part of this engine's path: returns ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:77-78. */
+/* out[b][i] = sum_t w[n_t] ifft(spec[b][:, t])[n_t] / sum_t w[n_t]^2 with n_t = nFft/2 + i - t*hop
+ * (overlap-add of the windowed inverse frames, squared-window envelope, centre crop); output
+ * complex64 (channel, nFft + hop*(nFrames-1) - 2*(nFft/2)), window float (nFft).  Shape or dtype
+ * not the plan's -> ACL_ERROR_INVALID_PARAM; null data -> ACL_ERROR_INTERNAL_ERROR; the output
+ * must not overlap the input or window (no in-place update).
+ * Replaces include/fft_api.h:77-78 (core/fft/istft.cpp:224-316, 437-523, 572-595). */
 AspbStatus asdFftExecIstft(asdFftHandle handle, const aclTensor* input, const aclTensor* windowOpt,
                            const aclTensor* output);
 

@@ -321,7 +321,7 @@ double oracle_rel_l2(const float* got, const double* golden, long count)
 /* ------------------------------------------------------------------------- */
 /* Batched 2D circular shift (include/fft_api.h:97-122 contract of          */
 /* asdFftShift / asdIfftShift; goldens in                                   */
-/* tests/ut/unittest/fft/{fftshift,ifftshift}/*_data/gen_data.py):           */
+/* tests/ut/unittest/fft/{fftshift,ifftshift}/<case>_data/gen_data.py):           */
 /*   out[s][(r + sr) % row][(c + sc) % col] = scale * in[s*dist + r*col + c] */
 /* with (sr, sc) = (ceil(row/2), ceil(col/2)) for the forward shift and     */
 /* (floor(row/2), floor(col/2)) for the inverse; the scale is applied in    */
@@ -344,4 +344,57 @@ void oracle_shift2d(const float* in, float* out, long row, long col, long batch,
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------------- */
+/* ISTFT (asdFftIstftMakePlan / asdFftExecIstft; the only supported form:    */
+/* center = true, normalized = false, onesided = false, length = 0,         */
+/* returnComplex = true), following core/fft/istft.cpp:288-433 and          */
+/* core/fft/fftcore/fft_core_istft_any.cpp (Run: 1/nfft, inverse DFT of each */
+/* frame, multiply by the window, overlap-add at stride hop, divide by the  */
+/* overlap-added squared window, keep [nfft/2, nfft/2 + out_len)), the      */
+/* golden of tests/ut/unittest/fft/istft/istft_data/gen_data_istft.py:37-64.*/
+/*   spec: complex64 [channel][nfft][nframes]   window: float32 [nfft]      */
+/*   out:  complex128 [channel][out_len], out_len = nfft + hop*(nframes-1)  */
+/*         - 2*(nfft/2)                                                      */
+/* ------------------------------------------------------------------------- */
+void oracle_istft(const float* spec, const float* window, double* out, long channel, long nfft, long nframes,
+                  long hop)
+{
+    const long full = nfft + hop * (nframes - 1);
+    const long start = nfft / 2;
+    const long out_len = full - 2 * (nfft / 2);
+    double* col = (double*)malloc(sizeof(double) * 2 * nfft);
+    double* frame = (double*)malloc(sizeof(double) * 2 * nfft);
+    double* y = (double*)malloc(sizeof(double) * 2 * full);
+    double* env = (double*)malloc(sizeof(double) * full);
+    for (long m = 0; m < full; m++) env[m] = 0.0;
+    for (long t = 0; t < nframes; t++) {
+        for (long n = 0; n < nfft; n++) env[t * hop + n] += (double)window[n] * (double)window[n];
+    }
+    for (long b = 0; b < channel; b++) {
+        for (long m = 0; m < 2 * full; m++) y[m] = 0.0;
+        for (long t = 0; t < nframes; t++) {
+            for (long k = 0; k < nfft; k++) {  /* frequency column t of channel b */
+                const float* src = spec + 2 * ((b * nfft + k) * nframes + t);
+                col[2 * k] = src[0];
+                col[2 * k + 1] = src[1];
+            }
+            oracle_fft(col, frame, nfft, +1);
+            for (long n = 0; n < nfft; n++) {
+                const double w = (double)window[n] / (double)nfft;
+                y[2 * (t * hop + n)] += frame[2 * n] * w;
+                y[2 * (t * hop + n) + 1] += frame[2 * n + 1] * w;
+            }
+        }
+        for (long i = 0; i < out_len; i++) {
+            const double e = env[start + i];
+            out[2 * (b * out_len + i)] = y[2 * (start + i)] / e;
+            out[2 * (b * out_len + i) + 1] = y[2 * (start + i) + 1] / e;
+        }
+    }
+    free(col);
+    free(frame);
+    free(y);
+    free(env);
 }

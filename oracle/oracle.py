@@ -11,6 +11,9 @@ Semantics (reference citations in fft_oracle.c):
   c2c_nd(x, ndim, inverse)   fftn over the trailing ndim axes
   r2c_nd(x, ndim, inverse)   R2C on the last axis, then C2C on the other FFT axes
   c2r_nd(x, dims, inverse)   C2C on the leading FFT axes, then C2R on the last
+  shift2d(...)               fftshift / ifftshift(+scale) of 2D slices
+  istft(spec, window, hop)   ISTFT: 1/nfft inverse DFT per frame, window, overlap-add,
+                             squared-window envelope, centre crop
 """
 from __future__ import annotations
 
@@ -48,6 +51,7 @@ def _load():
                                            fp, ctypes.c_long]
         lib.oracle_shift2d.argtypes = [fp, fp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        lib.oracle_istft.argtypes = [fp, fp, dp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long]
         lib.oracle_rel_l2.argtypes = [fp, dp, ctypes.c_long]
         lib.oracle_rel_l2.restype = ctypes.c_double
         _lib = lib
@@ -138,6 +142,25 @@ def shift2d(x: np.ndarray, row: int, col: int, batch: int, dist: int = 0, invers
     apply = 1 if (inverse and scale != 1.0) else 0
     _load().oracle_shift2d(_ptr(src.view(np.float32), ctypes.c_float), _ptr(out.view(np.float32), ctypes.c_float),
                            row, col, batch, dist, int(inverse), float(scale), apply)
+    return out
+
+
+def istft_out_len(nfft: int, hop: int, nframes: int) -> int:
+    """Output samples per channel of the ISTFT (center = true, length = 0), istft.cpp:318-325."""
+    return nfft + hop * (nframes - 1) - 2 * (nfft // 2)
+
+
+def istft(spec: np.ndarray, window: np.ndarray, hop: int) -> np.ndarray:
+    """ISTFT of a complex64 (channel, nfft, nframes) spectrum with a float32 window of nfft
+    samples (center, not normalised, two-sided); returns complex128 (channel, out_len)."""
+    spec = np.ascontiguousarray(spec, dtype=np.complex64)
+    window = np.ascontiguousarray(window, dtype=np.float32)
+    channel, nfft, nframes = spec.shape
+    if window.shape != (nfft,):
+        raise ValueError("istft: window must have nfft samples")
+    out = np.empty((channel, istft_out_len(nfft, hop, nframes)), dtype=np.complex128)
+    _load().oracle_istft(_ptr(spec.view(np.float32), ctypes.c_float), _ptr(window, ctypes.c_float),
+                         _ptr(out.view(np.float64), ctypes.c_double), channel, nfft, nframes, hop)
     return out
 
 

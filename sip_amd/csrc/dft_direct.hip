@@ -13,22 +13,23 @@ namespace asdfft {
 __device__ __forceinline__ float2 dft_load(const DftArgs& a, int64_t base, int i)
 {
     const float2* in2 = reinterpret_cast<const float2*>(a.in);
+    const int64_t s = a.in_stride;
     float2 x;
     if (a.prologue == kProReal) {
-        x = make_float2(reinterpret_cast<const float*>(a.in)[base + i], 0.0f);
+        x = make_float2(reinterpret_cast<const float*>(a.in)[base + i * s], 0.0f);
     } else if (a.prologue == kProPlanar) {
-        x = make_float2(reinterpret_cast<const float*>(a.in)[base + i], a.in_im[base + i]);
+        x = make_float2(reinterpret_cast<const float*>(a.in)[base + i * s], a.in_im[base + i * s]);
     } else if (a.prologue == kProHerm) {
         const int h = a.n >> 1;
         if (i <= h) {
-            x = in2[base + i];
+            x = in2[base + i * s];
             if (i == 0 || 2 * i == a.n) x.y = 0.0f;
         } else {
-            x = in2[base + (a.n - i)];
+            x = in2[base + (a.n - i) * s];
             x.y = -x.y;
         }
     } else {
-        x = in2[base + i];
+        x = in2[base + i * s];
     }
     if (a.conj_in) x.y = -x.y;
     return x;
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(256) dft_direct_kernel(const DftArgs a)
         const int tt = e / n;
         const int i = e - tt * n;
         const int64_t t = (int64_t)blockIdx.x * a.tpb + tt;
-        buf[e] = (t < a.count) ? dft_load(a, t * a.in_dist, i) : make_float2(0.0f, 0.0f);
+        buf[e] = (t < a.count) ? dft_load(a, (t % a.in_cols) + (t / a.in_cols) * a.in_dist, i) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
     const int nout = (a.epilogue == kEpiHalf) ? n / 2 + 1 : n;

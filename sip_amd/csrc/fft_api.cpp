@@ -35,6 +35,12 @@ constexpr int64_t kMaxFftSize = int64_t(1) << 27;  // fft_api.cpp:82
 
 enum class PlanState { kCreated, kInitialized, kFailed };
 
+// ISTFT plan parameters (IstftDesc, core/fft/istft.cpp): the spectrum shape recorded at plan
+// time and the overlap-add geometry.
+struct IstftDesc {
+    int64_t channel = 0, nfft = 0, nframes = 0, hop = 0, win_length = 0, out_len = 0;
+};
+
 struct FftPlan {
     std::mutex mu;
     PlanState state = PlanState::kCreated;
@@ -50,6 +56,7 @@ struct FftPlan {
     bool workspace_queried = false;
     asdfft::BuiltPlan built;
     std::unique_ptr<asdfft::RaggedPlan> ragged;  // set by asdFftExtMakePlanRagged
+    std::unique_ptr<IstftDesc> istft;            // set by asdFftIstftMakePlan
     asdfft::Precision precision = asdfft::Precision::kFp32;  // asdFftExtSetPrecision
 };
 
@@ -333,6 +340,10 @@ AspbStatus exec_common(asdFftHandle handle, const aclTensor* input, const aclTen
     }
     if (p.ragged) {
         log_error(fn, "ragged-batch plans run through asdFftExtExecRagged");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (p.istft) {
+        log_error(fn, "ISTFT plans run through asdFftExecIstft");
         return ACL_ERROR_INVALID_PARAM;
     }
     if (p.type != want) {  // fft_api.cpp:1764-1767
@@ -722,17 +733,183 @@ AspbStatus asdFftExecR2C(asdFftHandle handle, const aclTensor* input, const aclT
     return exec_common(handle, input, output, ASCEND_FFT_R2C, match_real, match_half_complex, "asdFftExecR2C");
 }
 
-AspbStatus asdFftIstftMakePlan(asdFftHandle, const aclTensor*, const int64_t, const int64_t, const int64_t, const bool,
-                               const bool, const bool, int64_t, const bool)
+AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor* input, const int64_t nFft,
+                               const int64_t hopLengthOpt, const int64_t winLengthOpt, const bool center,
+                               const bool normalized, const bool onesidedOpt, int64_t lengthOpt,
+                               const bool returnComplex)  // core/fft/istft.cpp:525-570
 {
-    log_error("asdFftIstftMakePlan", "ISTFT is not part of this engine");
-    return ACL_ERROR_API_NOT_SUPPORT;
+    const char* fn = "asdFftIstftMakePlan";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    // IstftParamsCheck (istft.cpp:343-433): the only supported form is center, not normalised,
+    // two-sided, full length, complex output; other flag values are replaced with a warning
+    if (input == nullptr || input->dtype != ACL_COMPLEX64) {
+        log_error(fn, "Invalid input or invalid input dtype! Only Support complex64!");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (input->view.size() != 3) {
+        log_error(fn, "Invalid input! Only 3-dimensional tensors are supported!");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    IstftDesc d;
+    d.channel = input->view[0];
+    const int64_t fft_size = input->view[1];
+    d.nframes = input->view[2];
+    if (!center) log_error(fn, "warning: Currently, the supported center is only true.");
+    if (normalized) log_error(fn, "warning: Currently, the supported normalized is only false.");
+    if (onesidedOpt) log_error(fn, "warning: Currently, the supported onesidedOpt is only false.");
+    if (!returnComplex) log_error(fn, "warning: Currently, the supported returnComplex is only true.");
+    if (d.nframes <= 1) {
+        log_error(fn, "Currently, the nFrames dim value of input should be more than 1");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (lengthOpt != 0) log_error(fn, "warning: Currently, the supported lengthOpt is only 0.");
+    if (nFft != fft_size) {
+        log_error(fn, "expected the frequency dimension of the input tensor to match n_fft when onesided=False");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!(0 < hopLengthOpt && hopLengthOpt <= winLengthOpt)) {
+        log_error(fn, "expected 0 < hop_length <= win_length");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!(winLengthOpt > 0 && winLengthOpt == nFft)) {
+        log_error(fn, "expected 0 < win_length == n_fft");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (nFft >= 1500) {
+        log_error(fn, "Currently, the nfft size should be less than 1500");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (hopLengthOpt >= 1500) {
+        log_error(fn, "Currently, the hoplen size should be less than 1500.");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    d.nfft = nFft;
+    d.hop = hopLengthOpt;
+    d.win_length = winLengthOpt;
+    d.out_len = nFft + hopLengthOpt * (d.nframes - 1) - 2 * (nFft / 2);
+    // MakePlan1DFft (istft.cpp:495-518): the frame transforms are a batch of channel * nframes
+    // inverse C2C; where that plan cannot be made the status is INTERNAL_ERROR
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    if (p.state == PlanState::kInitialized) {
+        log_error(fn, "plan is already initialized");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    const int64_t imax = std::numeric_limits<int32_t>::max();
+    if (d.channel <= 0 || d.nframes > imax / d.channel) {
+        log_error(fn, "channel * nFrames must be in [1, INT32_MAX]");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    asdfft::BuiltPlan built;
+    std::string why;
+    try {
+        why = asdfft::build_istft_frames(d.nfft, d.channel, d.nframes, &built);
+    } catch (const std::exception& e) {
+        why = e.what();
+    }
+    if (!why.empty()) {
+        log_error(fn, why);
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    p.type = ASCEND_FFT_C2C;
+    p.direction = ASCEND_FFT_INVERSE;
+    p.sizes = {nFft};
+    p.batch = d.channel * d.nframes;
+    p.vertical = false;
+    p.vertical_batch = 1;
+    p.built = std::move(built);
+    p.istft.reset(new IstftDesc(d));
+    p.state = PlanState::kInitialized;
+    if (trace_enabled()) fprintf(stderr, "[asdfft] %s", asdfft::describe_plan(p.built).c_str());
+    return ACL_SUCCESS;
 }
 
-AspbStatus asdFftExecIstft(asdFftHandle, const aclTensor*, const aclTensor*, const aclTensor*)
+AspbStatus asdFftExecIstft(asdFftHandle handle, const aclTensor* input, const aclTensor* windowOpt,
+                           const aclTensor* output)  // core/fft/istft.cpp:572-595, 248-316
 {
-    log_error("asdFftExecIstft", "ISTFT is not part of this engine");
-    return ACL_ERROR_API_NOT_SUPPORT;
+    const char* fn = "asdFftExecIstft";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    FftPlan& p = *sp;
+    std::lock_guard<std::mutex> lock(p.mu);
+    // MatchIstftC2C (istft.cpp:189-222): the spectrum shape recorded by the plan
+    const IstftDesc* d = p.istft.get();
+    if (d == nullptr || input == nullptr || input->dtype != ACL_COMPLEX64 || input->view.size() != 3 ||
+        input->view[0] != d->channel || input->view[1] != d->nfft || input->view[2] != d->nframes) {
+        log_error(fn, "input does not match c2c plan");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    // IstftOutTensorCheck, then IstftWindowTensorCheck (istft.cpp:437-523)
+    if (output == nullptr || output->dtype != ACL_COMPLEX64 || output->view.size() != 2 ||
+        output->view[0] != d->channel || output->view[1] != d->out_len) {
+        log_error(fn, "Invalid out tensor: expected complex64 (channel, " + std::to_string(d->out_len) + ")");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (windowOpt == nullptr || windowOpt->dtype != ACL_FLOAT || windowOpt->view.size() != 1 ||
+        windowOpt->view[0] != d->win_length || windowOpt->view[0] != d->nfft) {
+        log_error(fn, "Invalid window tensor: expected float (n_fft)");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    // asdFftExecIstftV2 (istft.cpp:224-286)
+    if (p.state != PlanState::kInitialized) {
+        log_error(fn, "plan is not initialized");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const AspbStatus ws = check_workspace(p, fn);
+    if (ws != ACL_SUCCESS) return ws;
+    char* in = static_cast<char*>(asdfft::data_address(input));
+    char* win = static_cast<char*>(asdfft::data_address(windowOpt));
+    char* out = static_cast<char*>(asdfft::data_address(output));
+    if (in == nullptr || win == nullptr || out == nullptr) {
+        log_error(fn, "input, window or output data is nullptr");
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    if (!asdfft::is_dense(input) || !asdfft::is_dense(windowOpt) || !asdfft::is_dense(output)) {
+        log_error(fn, "only dense row-major tensors are supported");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    // no in-place update (the reference's documented constraint); the output and the frames in
+    // the workspace must not overlap the operands either
+    const size_t in_bytes = (size_t)asdfft::numel(input) * 8;
+    const size_t out_bytes = (size_t)asdfft::numel(output) * 8;
+    const size_t win_bytes = (size_t)d->nfft * 4;
+    const char* work = static_cast<const char*>(p.workspace);
+    if (ranges_overlap(out, out_bytes, in, in_bytes) || ranges_overlap(out, out_bytes, win, win_bytes) ||
+        ranges_overlap(out, out_bytes, work, p.built.workspace) || ranges_overlap(in, in_bytes, work, p.built.workspace) ||
+        ranges_overlap(win, win_bytes, work, p.built.workspace)) {
+        log_error(fn, "output, input, window and workspace must not overlap");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    const AspbStatus dev = check_device(p.built, fn);
+    if (dev != ACL_SUCCESS) return dev;
+    hipStream_t stream = static_cast<hipStream_t>(p.stream);
+    char* bufs[3] = {in, out, static_cast<char*>(p.workspace)};
+    const AspbStatus st = launch_steps(p.built, bufs, nullptr, nullptr, stream, fn, -1);
+    if (st != ACL_SUCCESS) return st;
+    asdfft::OlaArgs a{};
+    a.frames = p.workspace;
+    a.window = reinterpret_cast<const float*>(win);
+    a.out = out;
+    a.nfft = (int)d->nfft;
+    a.hop = (int)d->hop;
+    a.nframes = (int)d->nframes;
+    a.start = (int)(d->nfft / 2);
+    a.channel = d->channel;
+    a.out_len = d->out_len;
+    a.scale = 1.0f / (float)d->nfft;
+    const hipError_t e = asdfft::launch_istft_ola(a, stream);
+    if (e != hipSuccess) {
+        log_error(fn, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        return ACL_ERROR_INTERNAL_ERROR;
+    }
+    return ACL_SUCCESS;
 }
 
 AspbStatus asdFftExecC2CSeparated(asdFftHandle handle, const aclTensor* inputReal, const aclTensor* inputImag,

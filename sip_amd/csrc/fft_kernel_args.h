@@ -43,6 +43,7 @@ enum Epilogue : int {
     kEpiTranspose = 5, // store transposed through LDS (four-step pass 2)
     kEpiPlanar = 6,    // real and imaginary parts to two float planes (out, out_im)
     kEpiTransposePlanar = 7,  // kEpiTranspose into two float planes
+    kEpiRowsOut = 8,   // column mapping: each transform stored contiguously through LDS
 };
 
 struct PassDesc {
@@ -117,6 +118,22 @@ struct DftArgs {
     int prologue, epilogue; // kProComplex/kProReal/kProHerm/kProPlanar, kEpiComplex/kEpiHalf/kEpiReal/kEpiPlanar
     int64_t count;
     int64_t in_dist, out_dist;
+    // input of transform t: (t % in_cols) + (t / in_cols) * in_dist, element i at + i * in_stride
+    // (1 and 1 for contiguous rows)
+    int64_t in_cols, in_stride;
+};
+
+// Overlap-add of inverse-transformed STFT frames (ISTFT): for output sample i of channel b,
+//   m = start + i,  y = sum_t frames[b][t][m - t*hop] * window[m - t*hop],
+//   env = sum_t window[m - t*hop]^2,  out[b][i] = y * scale / env
+// over the frames t with 0 <= m - t*hop < nfft.
+struct OlaArgs {
+    const void* frames;  // complex [channel][nframes][nfft]
+    const float* window; // [nfft]
+    void* out;           // complex [channel][out_len]
+    int nfft, hop, nframes, start;
+    int64_t channel, out_len;
+    float scale;         // 1 / nfft
 };
 
 }  // namespace asdfft

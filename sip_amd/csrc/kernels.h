@@ -25,6 +25,7 @@ enum Variant : int {
     kVarColsSepOut = 11,        // columns, complex in, planar out
     kVarColsTwiddleSep = 12,    // four-step pass 1 with a planar input
     kVarRowsTransposeSep = 13,  // four-step pass 2 with a planar output
+    kVarColsToRows = 14,        // strided columns in, each transform stored as a contiguous row
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a
@@ -56,6 +57,7 @@ extern const int kNumStaticInstances;
 // for the column mapping (rows keep transforms-per-workgroup at run time).
 int find_static(int variant, int emax, int n, int threads, int tpb, const int* radices, int nradices);
 hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStream_t stream);
+hipError_t launch_istft_ola(const OlaArgs& args, hipStream_t stream);
 
 }  // namespace asdfft
 
@@ -67,7 +69,8 @@ hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStrea
     X(0, 48) X(1, 48) X(2, 48) X(3, 48) X(4, 48) X(5, 48) \
     X(9, 16) X(10, 16) X(11, 16) X(12, 16) X(13, 16) \
     X(9, 32) X(10, 32) X(11, 32) X(12, 32) X(13, 32) \
-    X(9, 48) X(10, 48) X(11, 48)
+    X(9, 48) X(10, 48) X(11, 48) \
+    X(14, 16) X(14, 32)
 
 namespace asdfft {
 // True when the run-time kernel of (variant, emax) is compiled in.

@@ -192,7 +192,7 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
             const int T = threads_for(n, radices, emax);
             if (T == 0 || tpb * T > max_block_for(emax)) continue;
             const int64_t plane = n * (tpb + 1);  // column layout: row pitch tpb + 1 words
-            const bool need_lds = radices.size() > 1;
+            const bool need_lds = radices.size() > 1 || variant == kVarColsToRows;  // (epilogue via LDS)
             const size_t lds = need_lds ? (size_t)(8 * plane) : 0;
             if (lds > kLdsLimit) continue;
             c->radices = radices;
@@ -332,6 +332,8 @@ bool direct_step(int64_t n, int prologue, int epilogue, int64_t count, int64_t i
     d.count = count;
     d.in_dist = in_dist;
     d.out_dist = out_dist;
+    d.in_cols = 1;
+    d.in_stride = 1;
     st->direct = true;
     st->shape.variant = kVarDirect;
     st->shape.emax = 0;
@@ -794,6 +796,40 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
         first = false;
     }
     if (!real_rows(Kind::C2R, L, rows, inv, kBufWork, kBufOut, out, &why)) return why;
+    return "";
+}
+
+std::string build_istft_frames(int64_t nfft, int64_t channel, int64_t nframes, BuiltPlan* out)
+{
+    out->steps.clear();
+    out->workspace = 0;
+    if (nfft <= 0 || channel <= 0 || nframes <= 0) return "empty ISTFT input";
+    const int64_t count = channel * nframes;
+    std::string why;
+    Step st;
+    st.src = kBufIn;
+    st.dst = kBufWork;
+    st.bytes = 16.0 * (double)nfft * (double)count;
+    if (smooth_single(nfft)) {
+        // transform t = (b, frame): column `frame` of channel b's (nfft, nframes) block
+        StepIo io;
+        io.count = count;
+        io.cols = nframes;
+        io.in_inner = 1;
+        io.in_outer = nfft * nframes;
+        io.in_stride = nframes;
+        io.out_inner = nfft;
+        io.out_outer = nfft * nframes;
+        io.out_stride = 1;
+        if (!stockham_step(nfft, kVarColsToRows, true, io, 1, 1, 0, &st, &why)) return why;
+    } else {
+        if (!direct_step(nfft, kProComplex, kEpiComplex, count, nfft * nframes, nfft, 1, 1, &st, &why)) return why;
+        st.dargs.in_cols = nframes;
+        st.dargs.in_stride = nframes;
+    }
+    st.describe += " (ISTFT frames)";
+    out->steps.push_back(std::move(st));
+    out->workspace = (size_t)(count * nfft * 8);
     return "";
 }
 

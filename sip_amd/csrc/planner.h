@@ -101,6 +101,12 @@ struct BuiltPlan {
 // Returns "" on success, otherwise the reason the plan is unsupported.
 std::string build_plan(const PlanSpec& spec, BuiltPlan* out);
 
+// ISTFT frame transforms: unnormalised inverse C2C of length nfft along the strided frequency
+// axis of a (channel, nfft, nframes) complex spectrum, frame t of channel b stored as row
+// b * nframes + t of a (channel, nframes, nfft) complex workspace (one launch, src kBufIn,
+// dst kBufWork).  Returns "" on success, otherwise the reason.
+std::string build_istft_frames(int64_t nfft, int64_t channel, int64_t nframes, BuiltPlan* out);
+
 // Human-readable description of the steps (ASDFFT_PLAN_TRACE=1 prints it).
 std::string describe_plan(const BuiltPlan& plan);
 

@@ -383,7 +383,8 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
     constexpr int B = EMAX / R;
     static_assert(B >= 1, "radix larger than the per-thread register budget");
-    constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar);
+    constexpr bool kEpiViaLds =
+        (EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar || EPI == kEpiRowsOut);
     if constexpr (RT) {
         asm volatile("" : "+s"(g.n), "+s"(g.T));
         asm volatile("" : "+v"(tid));
@@ -630,6 +631,20 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                 }
             }
         }
+    } else if constexpr (EPI == kEpiRowsOut) {
+        // column mapping: LDS holds the workgroup's tpb transforms interleaved; each is written
+        // as one contiguous row (out_stride 1), consecutive lanes on consecutive elements
+        const int nthreads = g.T * g.tpb;
+        const int total = g.tpb * g.n;
+        for (int e = threadIdx.x; e < total; e += nthreads) {
+            const int tt2 = e / g.n;
+            const int i = e - tt2 * g.n;
+            const int64_t t2 = blk * g.tpb + tt2;
+            if (t2 >= a.count) break;  // e only grows: the rest of this thread's elements are past the end
+            const int64_t o = (t2 % a.cols) * a.out_inner + (t2 / a.cols) * a.out_outer;
+            const int at = lds_at<MAP>(g, tt2, i);
+            reinterpret_cast<float2*>(a.out)[o + i] = conj_if(make_float2(lre[at], lim[at]), io.cout);
+        }
     }
 }
 
@@ -748,6 +763,7 @@ template <> struct VariantTraits<kVarRowsSepIn> { static constexpr int map = kMa
 template <> struct VariantTraits<kVarColsSepOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiPlanar; };
 template <> struct VariantTraits<kVarColsTwiddleSep> { static constexpr int map = kMapCols, pro = kProPlanar, epi = kEpiTwiddle; };
 template <> struct VariantTraits<kVarRowsTransposeSep> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiTransposePlanar; };
+template <> struct VariantTraits<kVarColsToRows> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiRowsOut; };
 
 template <typename K>
 hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipStream_t stream)

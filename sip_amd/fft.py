@@ -165,6 +165,73 @@ class Plan:
             pass
 
 
+class Istft:
+    """One ISTFT plan (asdFftIstftMakePlan / asdFftExecIstft) for complex64 spectra of shape
+    (channel, nfft, nframes): center, not normalised, two-sided, complex output of shape
+    (channel, nfft + hop*(nframes-1) - 2*(nfft//2)).  Follows the reference's calling sequence
+    (tests/ut/unittest/fft/istft/test_fft_istft.cpp:118-150)."""
+
+    def __init__(self, channel: int, nfft: int, nframes: int, hop: int):
+        self.L = N.lib()
+        self.channel, self.nfft, self.nframes, self.hop = int(channel), int(nfft), int(nframes), int(hop)
+        self.out_len = self.nfft + self.hop * (self.nframes - 1) - 2 * (self.nfft // 2)
+        h = ctypes.c_void_p()
+        N.check("asdFftCreate", self.L.asdFftCreate(ctypes.byref(h)))
+        self.handle = h
+        shape = N.AclTensor((self.channel, self.nfft, self.nframes), N.ACL_COMPLEX64, 0)
+        try:
+            N.check("asdFftIstftMakePlan", self.L.asdFftIstftMakePlan(h, shape.ptr, self.nfft, self.hop, self.nfft,
+                                                                      True, False, False, 0, True))
+            size = ctypes.c_size_t(0)
+            N.check("asdFftGetWorkspaceSize", self.L.asdFftGetWorkspaceSize(h, ctypes.byref(size)))
+            self.workspace_size = size.value
+        except Exception:
+            self.L.asdFftDestroy(h)
+            self.handle = None
+            raise
+        finally:
+            shape.close()
+        self._workspace = None
+        self._stream = None
+
+    def describe(self) -> str:
+        need = ctypes.c_size_t(0)
+        self.L.asdFftExtDescribePlan(self.handle, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(need.value + 1)
+        self.L.asdFftExtDescribePlan(self.handle, buf, len(buf), None)
+        return buf.value.decode()
+
+    bind_workspace = Plan.bind_workspace
+    bind_stream = Plan.bind_stream
+
+    def execute(self, spec, window, out=None):
+        """spec: complex64 (channel, nfft, nframes) device tensor; window: float32 (nfft,)."""
+        import torch
+        if out is None:
+            out = torch.empty((self.channel, self.out_len), dtype=torch.complex64, device=spec.device)
+        self.bind_workspace(spec.device)
+        self.bind_stream(torch.cuda.current_stream(spec.device).cuda_stream)
+        ts = [N.AclTensor(t.shape, N.ACL_FLOAT if t.dtype == torch.float32 else N.ACL_COMPLEX64, t.data_ptr())
+              for t in (spec, window, out)]
+        try:
+            N.check("asdFftExecIstft", self.L.asdFftExecIstft(self.handle, *[t.ptr for t in ts]))
+        finally:
+            for t in ts:
+                t.close()
+        return out
+
+    def close(self):
+        if self.handle is not None:
+            self.L.asdFftDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _side_bytes(kind: str, n: int):
     """(input bytes, output bytes) of one transform of this kind and length."""
     h = n // 2 + 1

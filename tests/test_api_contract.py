@@ -274,10 +274,11 @@ def test_unsupported_lengths_rejected(L):
         assert L.asdFftMakePlan1D(h, 1009, C2C, FWD, 1, H) == OK  # prime handled by the direct DFT
 
 
-def test_out_of_scope_entries_report_not_supported(L):
+def test_next_entries_reject_null_tensors(L):
     with Handle(L) as h:
-        assert L.asdFftExecIstft(h, None, None, None) == NOT_SUPPORTED
-    # separated planes are implemented (tests/test_separated_contract.py)
+        # separated planes and ISTFT are implemented (tests/test_separated_contract.py,
+        # tests/test_istft_contract.py): null tensors are invalid parameters
+        assert L.asdFftExecIstft(h, None, None, None) == INVALID
     # fftshift / ifftshift are implemented (tests/test_shift_contract.py): null tensors are invalid
     assert L.asdFftShift(4, 4, 1, 0, None, None, None, None) == INVALID
     assert L.asdIfftShift(4, 4, 1, 0, 1.0, None, None, None, None) == INVALID

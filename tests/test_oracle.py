@@ -217,3 +217,85 @@ def test_shift_planted_mistake_is_caught(oracle):
     flat = np.arange(7 * 9, dtype=np.float32).astype(np.complex64)
     fwd = oracle.shift2d(flat, 7, 9, 1, 0, inverse=False)
     assert not np.array_equal(fwd, np.roll(flat.reshape(1, 7, 9), (3, 4), axis=(1, 2)))
+
+
+# ---- ISTFT (asdFftIstftMakePlan / asdFftExecIstft) ---------------------------------------------
+def _istft_window(n):
+    """The reference unit test's always-positive window (test_fft_istft.cpp:42-50)."""
+    return (0.6 + 0.4 * np.sin(2.0 * np.pi * np.arange(n) / n + 0.7)).astype(np.float32)
+
+
+def _torch_istft(spec, window, hop):
+    torch = pytest.importorskip("torch")
+    n = spec.shape[1]
+    return torch.istft(torch.from_numpy(spec.astype(np.complex128)), n_fft=n, hop_length=hop, win_length=n,
+                       window=torch.from_numpy(window.astype(np.float64)), center=True, normalized=False,
+                       onesided=False, return_complex=True).numpy()
+
+
+# (channel, nfft, nframes, hop): the reference's UT cases (test_fft_istft.cpp:205-213), odd and
+# prime lengths, no overlap (hop = nfft) and hop 1
+ISTFT_CASES = [(2, 64, 8, 16), (1, 128, 6, 32), (3, 256, 5, 64), (1, 128, 4, 16), (2, 63, 9, 7), (1, 97, 6, 20),
+               (2, 60, 5, 60), (1, 16, 20, 1), (1, 2, 3, 1)]
+
+
+@pytest.mark.parametrize("channel,nfft,nframes,hop", ISTFT_CASES)
+def test_istft_matches_torch_istft(oracle, channel, nfft, nframes, hop):
+    """Pinned on torch.istft (center, not normalised, two-sided, complex) in FP64: the library
+    routine whose semantics the reference's golden (gen_data_istft.py) restates."""
+    rng = np.random.default_rng(nfft * 10 + hop)
+    spec = (rng.standard_normal((channel, nfft, nframes)) +
+            1j * rng.standard_normal((channel, nfft, nframes))).astype(np.complex64)
+    w = _istft_window(nfft)
+    got = oracle.istft(spec, w, hop)
+    want = _torch_istft(spec, w, hop)
+    assert got.shape == (channel, oracle.istft_out_len(nfft, hop, nframes)) == want.shape
+    assert _rel(got, want) < 1e-13
+
+
+def test_istft_length_one_closed_form(oracle):
+    """nfft = 1 (hop 1): every frame is one sample, so out[b][i] = spec[b][0][i] / w[0]."""
+    spec = (np.arange(2 * 5) + 1j * np.arange(2 * 5)[::-1]).astype(np.complex64).reshape(2, 1, 5)
+    w = np.array([0.8], dtype=np.float32)
+    got = oracle.istft(spec, w, 1)
+    assert got.shape == (2, 5)
+    assert _rel(got, spec[:, 0, :].astype(np.complex128) / np.float64(w[0])) < 1e-15
+
+
+def test_istft_inverts_the_centred_stft(oracle):
+    """Round trip of the reference golden's own self-check (gen_data_istft.py:67-95): a signal's
+    zero-padded, windowed STFT comes back as the signal."""
+    nfft, hop = 128, 32
+    w = _istft_window(nfft).astype(np.float64)
+    rng = np.random.default_rng(42)
+    sig_len = nfft + hop * 5
+    t = np.arange(sig_len)
+    signal = 0.5 * np.sin(2 * np.pi * 3.0 * t / nfft) + 0.3 * rng.standard_normal(sig_len)
+    nframes = 1 + sig_len // hop
+    padded = np.pad(signal, (nfft // 2, nfft // 2))
+    frames = np.stack([padded[i * hop:i * hop + nfft] for i in range(nframes)], axis=1)
+    spec = np.fft.fft(frames * w[:, None], axis=0)[None, :, :].astype(np.complex64)
+    restored = oracle.istft(spec, w.astype(np.float32), hop)[0]
+    ref = signal[:restored.shape[0]]
+    assert np.max(np.abs(restored - ref)) / np.max(np.abs(ref)) < 1e-5
+
+
+def test_istft_planted_mistakes_are_caught(oracle):
+    """The plausible mistakes (forward sign, crop off by one sample, envelope left out) all fail
+    the torch.istft comparison."""
+    channel, nfft, nframes, hop = 2, 64, 8, 16
+    rng = np.random.default_rng(3)
+    spec = (rng.standard_normal((channel, nfft, nframes)) +
+            1j * rng.standard_normal((channel, nfft, nframes))).astype(np.complex64)
+    w = _istft_window(nfft)
+    want = _torch_istft(spec, w, hop)
+    good = oracle.istft(spec, w, hop)
+    assert _rel(good, want) < 1e-13
+    wrong_sign = np.conj(oracle.istft(np.conj(spec), w, hop))
+    assert _rel(wrong_sign, want) > 0.1
+    assert _rel(good[:, 1:], want[:, :-1]) > 0.1
+    env = np.zeros(nfft + hop * (nframes - 1))
+    for t in range(nframes):
+        env[t * hop:t * hop + nfft] += w.astype(np.float64) ** 2
+    no_env = good * env[nfft // 2:nfft // 2 + good.shape[1]]
+    assert _rel(no_env, want) > 0.1
