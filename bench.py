@@ -453,6 +453,53 @@ def bench_separated(torch, dev, stream, reps):
     return res
 
 
+def bench_istft(torch, dev, stream, reps):
+    """ISTFT form: asdFftExecIstft on a complex64 (channel, nfft, nframes) spectrum (center,
+    two-sided, complex output), event-timed through the C ABI, beside torch.istft (rocFFT) on
+    the same data.  Algorithmic bytes = spectrum read + window read + signal written."""
+    from sip_amd import _native as N
+    from sip_amd.fft import Istft
+    L = N.lib()
+    res = {}
+    for channel, nfft, nframes, hop in ((16, 1024, 1000, 256), (64, 400, 500, 160)):
+        g = torch.Generator(device=dev)
+        g.manual_seed(nfft + hop)
+        spec = torch.complex(torch.rand(channel, nfft, nframes, generator=g, device=dev) * 2 - 1,
+                             torch.rand(channel, nfft, nframes, generator=g, device=dev) * 2 - 1).contiguous()
+        k = torch.arange(nfft, device=dev, dtype=torch.float64)
+        win = (0.6 + 0.4 * torch.sin(2 * math.pi * k / nfft + 0.7)).float()
+        with torch.cuda.stream(stream):
+            p = Istft(channel, nfft, nframes, hop)
+            out = torch.empty((channel, p.out_len), dtype=torch.complex64, device=dev)
+            p.bind_workspace(dev)
+            p.bind_stream(stream.cuda_stream)
+            ts = [N.AclTensor(t.shape, N.ACL_FLOAT if t.dtype == torch.float32 else N.ACL_COMPLEX64, t.data_ptr())
+                  for t in (spec, win, out)]
+
+            def run():
+                st = L.asdFftExecIstft(p.handle, *[t.ptr for t in ts])
+                if st != 0:
+                    raise RuntimeError(f"asdFftExecIstft returned {st}")
+            ms = event_time_ms(torch, stream, run, reps)
+            ms_torch = event_time_ms(torch, stream, lambda: torch.istft(
+                spec, n_fft=nfft, hop_length=hop, win_length=nfft, window=win, center=True, normalized=False,
+                onesided=False, return_complex=True), reps)
+            ref = torch.istft(spec, n_fft=nfft, hop_length=hop, win_length=nfft, window=win, center=True,
+                              normalized=False, onesided=False, return_complex=True)
+            torch.cuda.synchronize()
+            rel = (torch.linalg.vector_norm(out - ref) / torch.linalg.vector_norm(ref)).item()
+            for t in ts:
+                t.close()
+            p.close()
+        alg = 8.0 * spec.numel() + 4.0 * nfft + 8.0 * out.numel()
+        res[f"istft_c{channel}_n{nfft}_t{nframes}_h{hop}"] = {
+            "us": round(ms * 1e3, 2), "alg_GBps": round(alg / ms / 1e6, 1),
+            "hbm_frac": round(alg / ms / 1e6 / HBM_PEAK_GBS, 4), "launches": 2,
+            "torch_istft_us": round(ms_torch * 1e3, 2), "speedup_vs_torch_istft": round(ms_torch / ms, 2),
+            "rel_l2_vs_torch_istft": float(f"{rel:.3e}")}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -468,6 +515,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the precision form")
     ap.add_argument("--no-separated", action="store_true", help="skip the separated-planes form")
     ap.add_argument("--separated-only", action="store_true", help="run only the separated-planes form")
+    ap.add_argument("--no-istft", action="store_true", help="skip the ISTFT form")
+    ap.add_argument("--istft-only", action="store_true", help="run only the ISTFT form")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
@@ -491,6 +540,9 @@ def main():
         return
     if a.separated_only:
         print(json.dumps({"forms": {"separated": bench_separated(torch, dev, stream, max(5, a.kernel_reps))}}))
+        return
+    if a.istft_only:
+        print(json.dumps({"forms": {"istft": bench_istft(torch, dev, stream, max(5, a.kernel_reps))}}))
         return
     L = N.lib()
     entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
@@ -607,6 +659,8 @@ def main():
         forms["precision"] = bench_precision(torch, dev, stream, max(5, a.kernel_reps))
     if not a.no_separated:
         forms["separated"] = bench_separated(torch, dev, stream, max(5, a.kernel_reps))
+    if not a.no_istft:
+        forms["istft"] = bench_istft(torch, dev, stream, max(5, a.kernel_reps))
     if forms:
         line["forms"] = forms
     if not a.no_cpu:
