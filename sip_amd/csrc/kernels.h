@@ -26,6 +26,7 @@ enum Variant : int {
     kVarColsTwiddleSep = 12,    // four-step pass 1 with a planar input
     kVarRowsTransposeSep = 13,  // four-step pass 2 with a planar output
     kVarColsToRows = 14,        // strided columns in, each transform stored as a contiguous row
+    kVarColsC2CSplit = 15,      // columns C2C exchanging through one LDS plane (specialised kernels only)
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a

@@ -102,6 +102,7 @@ struct Config {
     int64_t tstride = 0;  // rows mapping
     int64_t plane = 0;
     size_t lds = 0;
+    bool split = false;  // one-plane LDS exchange (kVarColsC2CSplit)
 };
 
 // Rows mapping: lanes along a transform; several short transforms per workgroup.
@@ -207,6 +208,34 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
         }
         if (found) break;
     }
+    // Long columns (2048) whose two LDS planes leave one workgroup per CU: a specialised kernel
+    // exchanging through one plane fits two workgroups of 8 columns per CU (measured 55 vs 70 us
+    // for the 2048 x 2048 x 4 column pass; 16 columns on one plane, one workgroup per CU, 62 us).
+    // ASDFFT_SPLIT_EMAX=0 disables it; ASDFFT_SPLIT_EMAX / ASDFFT_SPLIT_TPB pick the register
+    // budget / columns per workgroup of a specialised instance (measurement).
+    const char* ns = getenv("ASDFFT_NO_STATIC");
+    const char* se = getenv("ASDFFT_SPLIT_EMAX");
+    const int split_only = se ? atoi(se) : -1;
+    const char* st = getenv("ASDFFT_SPLIT_TPB");
+    const int split_tpb = st ? atoi(st) : 8;
+    if (variant == kVarColsC2C && found && c->tpb <= split_tpb && count >= split_tpb && split_only != 0 &&
+        c->lds > (size_t)n * (split_tpb + 1) * 4 && !(ns != nullptr && ns[0] == '1')) {
+        for (int emax : {32, 64}) {
+            if (split_only > 0 && emax != split_only) continue;
+            const int T = threads_for(n, radices, emax);
+            const size_t lds = (size_t)n * (split_tpb + 1) * 4;
+            if (T == 0 || split_tpb * T > 1024 || lds > kLdsLimit) continue;
+            if (find_static(kVarColsC2CSplit, emax, (int)n, T, split_tpb, radices.data(), (int)radices.size()) < 0)
+                continue;
+            c->emax = emax;
+            c->threads = T;
+            c->tpb = split_tpb;
+            c->plane = n * (split_tpb + 1);
+            c->lds = lds;
+            c->split = true;
+            break;
+        }
+    }
     return found;
 }
 
@@ -278,7 +307,7 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     a.xcd_remap = xcd_remap_for(cols_mapping, c.tpb);
     fill_passes(c, &a, &offsets);
     st->direct = false;
-    st->shape.variant = variant;
+    st->shape.variant = c.split ? (int)kVarColsC2CSplit : variant;
     st->shape.emax = c.emax;
     st->shape.block = c.tpb * c.threads;
     st->shape.lds_bytes = c.lds;
@@ -286,11 +315,11 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     st->shape.static_id = -1;
     const char* ns = getenv("ASDFFT_NO_STATIC");
     if (ns == nullptr || ns[0] != '1') {
-        st->shape.static_id = find_static(variant, c.emax, (int)n, c.threads, c.tpb, c.radices.data(),
+        st->shape.static_id = find_static(st->shape.variant, c.emax, (int)n, c.threads, c.tpb, c.radices.data(),
                                           (int)c.radices.size());
     }
     std::ostringstream os;
-    os << "stockham variant=" << variant << " n=" << n << " count=" << io.count << " radices=";
+    os << "stockham variant=" << st->shape.variant << " n=" << n << " count=" << io.count << " radices=";
     for (size_t i = 0; i < c.radices.size(); i++) os << (i ? "x" : "") << c.radices[i];
     os << " emax=" << c.emax << " T=" << c.threads << " tpb=" << c.tpb << " block=" << st->shape.block
        << " lds=" << c.lds << " grid=" << st->shape.grid;

@@ -29,6 +29,11 @@ struct IC {
     static constexpr int value = V;
 };
 
+template <int F, int... More>
+struct First {
+    static constexpr int value = F;
+};
+
 template <int Start, int End, typename F>
 __device__ __forceinline__ void static_for(F&& f)
 {
@@ -748,6 +753,121 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
     lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
 }
 
+// ------------------------------------------------------------------ one-plane exchange (columns)
+// Specialised column kernel whose passes exchange through ONE LDS plane: first the real
+// parts of every point, then the imaginary parts.  It needs half the LDS of the two-plane
+// exchange, so for long columns (2048 points, 8 columns: 72 KiB instead of 144 KiB) two
+// workgroups share a CU and one's loads overlap the other's passes; the cost is two more
+// barriers per exchange.  The
+// points stay in registers across passes (the exchange overwrites them plane by plane:
+// all real parts are stored before any is read back, and the imaginary parts, still in
+// the previous pass's layout, are stored only after that).
+template <int EMAX, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
+__device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt, int tid, bool live, const Io& io,
+                                             float2 (&v)[EMAX])
+{
+    constexpr int B = EMAX / R;
+    constexpr int NR = N / R;
+    constexpr int PITCH = TPB + 1;  // column layout: slot of element i of column tt = i * PITCH + tt
+    static_assert(B >= 1, "radix larger than the per-thread register budget");
+    float* lp_t = lp + tt;
+    if constexpr (P == 0) {
+        const float2* in2 = reinterpret_cast<const float2*>(io.in);
+        const int s = io.in_stride;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (live && j < NR) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b * R + r] = in2[(j + r * NR) * s];
+            }
+        }
+        if (io.cin) {
+#pragma unroll
+            for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
+        }
+    }
+    const float2* twp = reinterpret_cast<const float2*>(a.tw) + TWOFF;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+        if (j < NR) {
+            if constexpr (SPAN > 1) {
+                const float2* w = twp + (j % SPAN) * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
+            }
+            Dft<R>::run(&v[b * R]);
+        }
+    }
+    if constexpr (sizeof...(Rest) == 0) {
+        if (io.cout) {
+#pragma unroll
+            for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
+        }
+        float2* out2 = reinterpret_cast<float2*>(io.out);
+        const int s = io.out_stride;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            if (live && j < NR) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) out2[(j + r * NR) * s] = v[b * R + r];
+            }
+        }
+    } else {
+        constexpr int RN = First<Rest...>::value;  // the next pass's radix
+        constexpr int BN = EMAX / RN;
+        constexpr int NRN = N / RN;
+        static_for<0, 2>([&](auto PL) __attribute__((always_inline)) {
+            constexpr bool kIm = decltype(PL)::value == 1;
+            __syncthreads();  // the previous plane's reads are done before it is overwritten
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = tid + b * T;
+                if (j < NR) {
+                    const int k = (SPAN > 1) ? j % SPAN : 0;
+                    const int base = (j - k) * R + k;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) lp_t[(base + r * SPAN) * PITCH] = kIm ? v[b * R + r].y : v[b * R + r].x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < BN; ++b) {
+                const int j = tid + b * T;
+                if (j < NRN) {
+#pragma unroll
+                    for (int r = 0; r < RN; ++r) {
+                        const float x = lp_t[(j + r * NRN) * PITCH];
+                        if constexpr (kIm) {
+                            v[b * RN + r].y = x;
+                        } else {
+                            v[b * RN + r].x = x;
+                        }
+                    }
+                }
+            }
+        });
+        constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
+        split_passes<EMAX, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, lp, tt, tid, live, io, v);
+    }
+}
+
+template <int EMAX, int N, int T, int TPB, int... Rs>
+__global__ void __launch_bounds__(T * TPB) stockham_split_kernel(const FftArgs a)
+{
+    extern __shared__ float smem[];
+    const Geo g{N, T, TPB, 0};
+    int tt, tid;
+    int64_t blk;
+    bool live;
+    Io io;
+    thread_setup<kMapCols, kProComplex, kEpiComplex>(a, g, tt, tid, blk, live, io);
+    float2 v[EMAX];
+    split_passes<EMAX, N, T, TPB, 0, 1, 0, Rs...>(a, smem, tt, tid, live, io, v);
+}
+
 // ------------------------------------------------------------------ launch helper
 template <int V> struct VariantTraits;
 template <> struct VariantTraits<kVarRowsC2C> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiComplex; };
@@ -787,8 +907,13 @@ hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t
 template <int V, int EMAX, int N, int T, int TPB, int... Rs>
 hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
-    using Tr = VariantTraits<V>;
-    return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args, stream);
+    if constexpr (V == kVarColsC2CSplit) {
+        return launch_kernel(stockham_split_kernel<EMAX, N, T, TPB, Rs...>, s, args, stream);
+    } else {
+        using Tr = VariantTraits<V>;
+        return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args,
+                             stream);
+    }
 }
 
 }  // namespace asdfft

@@ -290,3 +290,39 @@ def test_specialised_kernels_match_runtime_kernel(torch_dev, oracle, monkeypatch
                                else outs[1].astype(np.float64), len(dims)).max() <= 1e-6
             checked += 1
     assert checked >= 20
+
+
+# 2048-point column passes (the one-plane LDS exchange kernel, 16 or 8 columns per
+# workgroup): full and partial workgroups, 2D, 3D and VERTICAL 1D
+LONG_COLUMNS = [("c2c", (2048, 64), 1, False), ("c2c", (2048, 40), 2, True), ("c2c", (2048, 4, 8), 1, False),
+                ("c2c", (2048, 24), 1, True)]
+
+
+@pytest.mark.parametrize("kind,dims,batch,inverse", LONG_COLUMNS, ids=str)
+def test_long_column_passes(torch_dev, oracle, kind, dims, batch, inverse):
+    rng = np.random.default_rng(sum(dims) + batch)
+    shape = (batch,) + dims
+    x = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+    got = run_gpu(torch_dev, kind, dims, batch, inverse, x)
+    want = oracle.c2c_nd(x, len(dims), inverse)
+    err = per_row_rel(got, want, len(dims)).max()
+    assert err <= tol_for(dims), (err, tol_for(dims))
+
+
+def test_vertical_2048_columns(torch_dev, oracle):
+    import torch
+
+    from sip_amd.fft import Plan
+    n, batch = 2048, 20
+    rng = np.random.default_rng(20)
+    x = (rng.uniform(-1, 1, (n, batch)) + 1j * rng.uniform(-1, 1, (n, batch))).astype(np.complex64)
+    p = Plan("c2c", (n,), batch, False, vertical=True)
+    try:
+        y = p.execute(torch.from_numpy(x).to(torch_dev))
+        torch.cuda.synchronize()
+        got = y.cpu().numpy()
+    finally:
+        p.close()
+    want = oracle.c2c_nd(np.ascontiguousarray(x.T), 1, False).T
+    err = per_row_rel(np.ascontiguousarray(got.T), np.ascontiguousarray(want.T), 1).max()
+    assert err <= tol_for((n,)), err

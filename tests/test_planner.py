@@ -61,7 +61,12 @@ def test_multidim_plans_have_no_transposes():
     p = Plan("c2c", (2048, 2048), 4)
     s = steps(p.describe())
     assert len(s) == 2 and "[in->out]" in s[0] and "[out->out]" in s[1]
-    assert field(s[1], "variant") == "1"  # strided column pass
+    # strided column pass: the 2048-point columns exchange through one LDS plane (variant 15),
+    # 8 columns per workgroup in 72 KiB, two workgroups per CU
+    assert field(s[1], "variant") == "15" and field(s[1], "tpb") == "8" and field(s[1], "lds") == "73728"
+    p.close()
+    p = Plan("c2c", (1024, 1024), 4)  # two planes of 16 columns fit: the ordinary column kernel
+    assert field(steps(p.describe())[1], "variant") == "1"
     p.close()
     p = Plan("c2c", (256, 256, 256), 1)
     assert len(steps(p.describe())) == 3 and p.workspace_size == 0
