@@ -89,6 +89,27 @@ int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t* out_exten
  * unknown mode or a handle whose plan already exists. */
 int32_t asdFftExtSetPrecision(void* handle, int32_t mode);
 
+/* ---- shifts folded into the transform ----------------------------------------------------
+ * ASDFFT_SHIFT_FFT is the permutation of asdFftShift (every FFT axis of length n rolled by
+ * ceil(n/2)), ASDFFT_SHIFT_IFFT that of asdIfftShift (rolled by floor(n/2)).  A plan made after
+ * asdFftExtSetShift(h, in_shift, out_shift, scale) computes
+ *     out = scale * out_shift( FFT( in_shift(in) ) )
+ * with the input permutation applied while the first pass reads and the output permutation
+ * and scale while the last pass writes: no extra pass over HBM (the separate asdIfftShift +
+ * exec + asdFftShift sequence makes two).  For 1D and 2D C2C plans whose axis lengths are
+ * 2..16384 with prime factors <= 19; MakePlan returns ACL_ERROR_API_NOT_SUPPORT for other
+ * plans.  2D plans need a workspace of batch*X*Y*8 bytes; exec refuses in-place calls
+ * (ACL_ERROR_INVALID_PARAM). */
+#define ASDFFT_SHIFT_NONE 0
+#define ASDFFT_SHIFT_FFT 1
+#define ASDFFT_SHIFT_IFFT 2
+
+/* No reference counterpart (fused shifts; callers that follow or precede a transform with
+ * asdFftShift / asdIfftShift).  Chooses the shifts and output scale of the plan that the next
+ * asdFftMakePlan1D / 2D call builds on this handle; ACL_ERROR_INVALID_PARAM for an unknown mode
+ * or a handle whose plan already exists. */
+int32_t asdFftExtSetShift(void* handle, int32_t in_shift, int32_t out_shift, double scale);
+
 /* ---- serving loop ------------------------------------------------------------------------
  * A server admits FFT requests in arrival order while their input + output bytes (plus a
  * plan workspace) fit an HBM budget, and runs them in batches: whenever one of `slots`

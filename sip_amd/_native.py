@@ -99,6 +99,9 @@ class ServeCounters(ctypes.Structure):
 
 ASDFFT_PRECISION_FP32 = 0
 ASDFFT_PRECISION_SPLIT16 = 1
+ASDFFT_SHIFT_NONE = 0
+ASDFFT_SHIFT_FFT = 1
+ASDFFT_SHIFT_IFFT = 2
 ASDFFT_RAGGED_GRAPH = 0x1
 ASDFFT_RAGGED_AUTOTUNE = 0x2
 
@@ -152,6 +155,7 @@ SIGNATURES = [
     ("asdFftExtRaggedInfo", _I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)]),
     ("asdFftExtSetPrecision", _I32, [_VP, _I32]),
+    ("asdFftExtSetShift", _I32, [_VP, _I32, _I32, ctypes.c_double]),
     ("asdFftExtServeCreate", _I32, [ctypes.POINTER(_VP), ctypes.POINTER(ServeConfig)]),
     ("asdFftExtServeSubmit", _I32, [_VP, ctypes.POINTER(ServeRequest), ctypes.POINTER(ctypes.c_uint64)]),
     ("asdFftExtServeWait", _I32, [_VP, ctypes.c_uint64, _I64, ctypes.POINTER(ctypes.c_double)]),

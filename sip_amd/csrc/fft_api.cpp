@@ -58,6 +58,10 @@ struct FftPlan {
     std::unique_ptr<asdfft::RaggedPlan> ragged;  // set by asdFftExtMakePlanRagged
     std::unique_ptr<IstftDesc> istft;            // set by asdFftIstftMakePlan
     asdfft::Precision precision = asdfft::Precision::kFp32;  // asdFftExtSetPrecision
+    int in_shift = asdfft::kShiftNone;                       // asdFftExtSetShift
+    int out_shift = asdfft::kShiftNone;
+    float out_scale = 1.0f;
+    bool shifted() const { return in_shift != asdfft::kShiftNone || out_shift != asdfft::kShiftNone || out_scale != 1.0f; }
 };
 
 std::mutex g_table_mu;
@@ -112,6 +116,9 @@ AspbStatus finish_plan(FftPlan& p, const char* fn)
     spec.vertical = p.vertical;
     spec.planar = (p.type == ASCEND_FFT_C2C_SEP);
     spec.precision = p.precision;
+    spec.in_shift = p.in_shift;
+    spec.out_shift = p.out_shift;
+    spec.out_scale = p.out_scale;
     asdfft::BuiltPlan built;
     std::string why;
     try {
@@ -122,7 +129,7 @@ AspbStatus finish_plan(FftPlan& p, const char* fn)
     if (!why.empty()) {
         log_error(fn, why);
         p.built = asdfft::BuiltPlan{};
-        return ACL_ERROR_INVALID_PARAM;
+        return p.shifted() ? ACL_ERROR_API_NOT_SUPPORT : ACL_ERROR_INVALID_PARAM;
     }
     p.built = std::move(built);
     p.state = PlanState::kInitialized;
@@ -312,7 +319,8 @@ AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output,
     }
     const size_t in_bytes = (size_t)asdfft::numel(input) * (size_t)asdfft::element_bytes(input->dtype);
     const size_t out_bytes = (size_t)asdfft::numel(output) * (size_t)asdfft::element_bytes(output->dtype);
-    const bool same = (in == out && in_bytes == out_bytes && p.type == ASCEND_FFT_C2C);
+    // exact in-place C2C is allowed, except with folded shifts (they move data between workgroups)
+    const bool same = (in == out && in_bytes == out_bytes && p.type == ASCEND_FFT_C2C && !p.shifted());
     if (!same && ranges_overlap(in, in_bytes, out, out_bytes)) {
         log_error(fn, "input and output overlap (only exact in-place C2C is supported)");
         return ACL_ERROR_INVALID_PARAM;
@@ -1199,6 +1207,38 @@ extern "C" int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t
     if (out_extent != nullptr) *out_extent = (int64_t)r.out_extent;
     if (bytes != nullptr) *bytes = r.bytes;
     if (flops != nullptr) *flops = r.flops;
+    return ACL_SUCCESS;
+}
+
+extern "C" int32_t asdFftExtSetShift(void* handle, int32_t in_shift, int32_t out_shift, double scale)
+{
+    const char* fn = "asdFftExtSetShift";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    std::lock_guard<std::mutex> lock(sp->mu);
+    if (sp->state == PlanState::kInitialized) {
+        log_error(fn, "shifts must be chosen before the plan is made");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    auto valid = [](int32_t m) { return m == ASDFFT_SHIFT_NONE || m == ASDFFT_SHIFT_FFT || m == ASDFFT_SHIFT_IFFT; };
+    if (!valid(in_shift) || !valid(out_shift)) {
+        log_error(fn, "unknown shift mode");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    if (!(scale == scale) || scale == std::numeric_limits<double>::infinity() ||
+        scale == -std::numeric_limits<double>::infinity()) {
+        log_error(fn, "scale must be finite");
+        return ACL_ERROR_INVALID_PARAM;
+    }
+    sp->in_shift = in_shift == ASDFFT_SHIFT_FFT ? asdfft::kShiftFft
+                                                 : in_shift == ASDFFT_SHIFT_IFFT ? asdfft::kShiftIfft : asdfft::kShiftNone;
+    sp->out_shift = out_shift == ASDFFT_SHIFT_FFT ? asdfft::kShiftFft
+                                                   : out_shift == ASDFFT_SHIFT_IFFT ? asdfft::kShiftIfft
+                                                                                    : asdfft::kShiftNone;
+    sp->out_scale = (float)scale;
     return ACL_SUCCESS;
 }
 

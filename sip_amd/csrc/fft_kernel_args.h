@@ -31,6 +31,7 @@ enum Prologue : int {
     kProHerm = 2,      // Hermitian half -> full spectrum (odd-length C2R)
     kProC2REven = 3,   // even-length C2R pre-twiddle: Z[k] from X[k], X[M-k]
     kProPlanar = 4,    // real and imaginary parts from two float planes (in, in_im)
+    kProShift = 5,     // complex load through the input shift (roll_in, in_o_mod / in_o_roll)
 };
 
 // How the last pass disposes of its outputs.
@@ -44,6 +45,7 @@ enum Epilogue : int {
     kEpiPlanar = 6,    // real and imaginary parts to two float planes (out, out_im)
     kEpiTransposePlanar = 7,  // kEpiTranspose into two float planes
     kEpiRowsOut = 8,   // column mapping: each transform stored contiguously through LDS
+    kEpiShift = 9,     // complex store through the output shift (roll_out, out_col_roll), times out_scale
 };
 
 struct PassDesc {
@@ -76,6 +78,14 @@ struct FftArgs {
     int64_t in_inner, in_outer, in_stride;
     int64_t out_inner, out_outer, out_stride;
     int64_t out_cols;       // transposed store: transforms per output group (kEpiTranspose)
+    // shifts folded into the first / last pass (kProShift, kEpiShift): element i of the
+    // transform is loaded from element (i - roll_in) mod n, and stored to (i + roll_out) mod n;
+    // the outer index o (t / cols) is read as (o - (o mod in_o_mod) + ((o mod in_o_mod) -
+    // in_o_roll) mod in_o_mod) (in_o_mod 1: none); the column index (t mod cols) is stored as
+    // (col + out_col_roll) mod cols; stored values are multiplied by out_scale
+    int roll_in, roll_out;
+    int64_t in_o_mod, in_o_roll, out_col_roll;
+    float out_scale;
     PassDesc pass[kMaxPasses];
 };
 

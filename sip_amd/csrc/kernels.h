@@ -27,6 +27,10 @@ enum Variant : int {
     kVarRowsTransposeSep = 13,  // four-step pass 2 with a planar output
     kVarColsToRows = 14,        // strided columns in, each transform stored as a contiguous row
     kVarColsC2CSplit = 15,      // columns C2C exchanging through one LDS plane (specialised kernels only)
+    // shifts folded into the transform (asdFftExtSetShift)
+    kVarRowsShiftIn = 16,       // rows, input shift
+    kVarRowsShiftBoth = 17,     // rows, input and output shift
+    kVarColsShiftOut = 18,      // columns, output shift
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a
@@ -71,7 +75,8 @@ hipError_t launch_istft_ola(const OlaArgs& args, hipStream_t stream);
     X(9, 16) X(10, 16) X(11, 16) X(12, 16) X(13, 16) \
     X(9, 32) X(10, 32) X(11, 32) X(12, 32) X(13, 32) \
     X(9, 48) X(10, 48) X(11, 48) \
-    X(14, 16) X(14, 32)
+    X(14, 16) X(14, 32) \
+    X(16, 16) X(17, 16) X(18, 16) X(16, 32) X(17, 32) X(18, 32) X(16, 48) X(17, 48)
 
 namespace asdfft {
 // True when the run-time kernel of (variant, emax) is compiled in.

@@ -305,6 +305,8 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     a.out_outer = io.out_outer;
     a.out_stride = io.out_stride;
     a.xcd_remap = xcd_remap_for(cols_mapping, c.tpb);
+    a.in_o_mod = 1;
+    a.out_scale = 1.0f;
     fill_passes(c, &a, &offsets);
     st->direct = false;
     st->shape.variant = c.split ? (int)kVarColsC2CSplit : variant;
@@ -703,6 +705,91 @@ int threads_for(int64_t n, const std::vector<int>& radices, int emax)
     return (int)t;
 }
 
+namespace {
+
+// Roll of an axis of length n for a shift mode: the forward shift (asdFftShift) moves element
+// k to (k + ceil(n/2)) mod n, the inverse shift (asdIfftShift) to (k + floor(n/2)) mod n.
+int shift_roll(int mode, int64_t n)
+{
+    if (mode == kShiftFft) return (int)((n + 1) / 2);
+    if (mode == kShiftIfft) return (int)(n / 2);
+    return 0;
+}
+
+// C2C with the shifts folded into the first and last pass (1D rows; 2D rows into the workspace,
+// then columns into the output: the output column roll moves data between workgroups, so the
+// column pass cannot run in place).
+std::string build_shift_plan(const PlanSpec& spec, BuiltPlan* out)
+{
+    std::string why;
+    if (spec.kind != Kind::C2C || spec.planar || spec.vertical) return "shifts are folded into C2C plans only";
+    const size_t nd = spec.dims.size();
+    if (nd != 1 && nd != 2) return "shifts are folded into 1D and 2D plans only";
+    for (int64_t d : spec.dims) {
+        if (d < 2 || !smooth_single(d)) return "shifts are folded only for axis lengths 2..16384 with prime factors <= 19";
+    }
+    const int64_t B = spec.batch;
+    const bool inv = spec.inverse;
+    if (nd == 1) {
+        const int64_t n = spec.dims[0];
+        Step st;
+        st.src = kBufIn;
+        st.dst = kBufOut;
+        st.bytes = 16.0 * (double)n * (double)B;
+        StepIo io;
+        io.count = B;
+        io.cols = 1;
+        io.in_outer = n;
+        io.out_outer = n;
+        if (!stockham_step(n, kVarRowsShiftBoth, false, io, inv, inv, 0, &st, &why)) return why;
+        st.args.roll_in = shift_roll(spec.in_shift, n);
+        st.args.roll_out = shift_roll(spec.out_shift, n);
+        st.args.out_scale = spec.out_scale;
+        st.describe += " (shift in " + std::to_string(st.args.roll_in) + " out " + std::to_string(st.args.roll_out) + ")";
+        out->steps.push_back(std::move(st));
+        return "";
+    }
+    const int64_t X = spec.dims[0], Y = spec.dims[1];
+    out->workspace = (size_t)(B * X * Y * 8);
+    Step r;
+    r.src = kBufIn;
+    r.dst = kBufWork;
+    r.bytes = 16.0 * (double)X * (double)Y * (double)B;
+    StepIo ir;
+    ir.count = B * X;
+    ir.cols = 1;
+    ir.in_outer = Y;
+    ir.out_outer = Y;
+    if (!stockham_step(Y, kVarRowsShiftIn, false, ir, inv, inv, 0, &r, &why)) return why;
+    r.args.roll_in = shift_roll(spec.in_shift, Y);
+    r.args.in_o_mod = X;
+    r.args.in_o_roll = shift_roll(spec.in_shift, X);
+    r.describe += " (shift in " + std::to_string(r.args.in_o_roll) + "," + std::to_string(r.args.roll_in) + ")";
+    Step c;
+    c.src = kBufWork;
+    c.dst = kBufOut;
+    c.bytes = r.bytes;
+    StepIo ic;
+    ic.count = B * Y;
+    ic.cols = Y;
+    ic.in_inner = 1;
+    ic.in_outer = X * Y;
+    ic.in_stride = Y;
+    ic.out_inner = 1;
+    ic.out_outer = X * Y;
+    ic.out_stride = Y;
+    if (!stockham_step(X, kVarColsShiftOut, true, ic, inv, inv, 0, &c, &why)) return why;
+    c.args.roll_out = shift_roll(spec.out_shift, X);
+    c.args.out_col_roll = shift_roll(spec.out_shift, Y);
+    c.args.out_scale = spec.out_scale;
+    c.describe += " (shift out " + std::to_string(c.args.roll_out) + "," + std::to_string(c.args.out_col_roll) + ")";
+    out->steps.push_back(std::move(r));
+    out->steps.push_back(std::move(c));
+    return "";
+}
+
+}  // namespace
+
 std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
 {
     out->steps.clear();
@@ -714,6 +801,9 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
     if (nd < 1 || nd > 3) return "unsupported rank";
 
     if (spec.planar && spec.kind != Kind::C2C) return "separated real/imaginary planes are C2C only";
+    if (spec.in_shift != kShiftNone || spec.out_shift != kShiftNone || spec.out_scale != 1.0f) {
+        return build_shift_plan(spec, out);
+    }
     if (nd == 1) {
         const int64_t n = spec.dims[0];
         if (spec.planar) {

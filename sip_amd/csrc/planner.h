@@ -83,6 +83,10 @@ struct Step {
     double bytes = 0.0;
 };
 
+// Shift folded into a plan (asdFftExtSetShift): the permutation of asdFftShift / asdIfftShift
+// on every FFT axis.
+enum ShiftMode : int { kShiftNone = 0, kShiftFft = 1, kShiftIfft = 2 };
+
 struct PlanSpec {
     Kind kind = Kind::C2C;
     bool inverse = false;
@@ -90,6 +94,9 @@ struct PlanSpec {
     int64_t batch = 1;
     bool vertical = false;      // 1D VERTICAL layout: tensor (N, batch)
     bool planar = false;        // C2C on separated real / imaginary float planes (C2C_SEP)
+    int in_shift = kShiftNone;  // applied to the input before the transform
+    int out_shift = kShiftNone; // applied to the transform's output
+    float out_scale = 1.0f;     // output multiplier (float32)
     Precision precision = Precision::kFp32;
 };
 

@@ -314,6 +314,10 @@ __device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int
         return make_float2(reinterpret_cast<const float*>(io.in)[i * s], 0.0f);
     } else if constexpr (PRO == kProPlanar) {
         return make_float2(reinterpret_cast<const float*>(io.in)[i * s], io.in_im[i * s]);  // conj_in by the caller
+    } else if constexpr (PRO == kProShift) {
+        int k = i - a.roll_in;  // element i of the shifted input is element i - roll of the input
+        if (k < 0) k += n;
+        return in2[k * s];      // conjugation (conj_in) is applied by the caller
     } else if constexpr (PRO == kProHerm) {
         const int h = n >> 1;
         float2 x;
@@ -356,6 +360,10 @@ __device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int
     } else if constexpr (EPI == kEpiPlanar) {
         reinterpret_cast<float*>(io.out)[i * s] = x.x;  // conj_out applied by the caller
         io.out_im[i * s] = x.y;
+    } else if constexpr (EPI == kEpiShift) {
+        int k = i + a.roll_out;
+        if (k >= n) k -= n;
+        out2[k * s] = make_float2(x.x * a.out_scale, x.y * a.out_scale);  // conj_out applied by the caller
     } else if constexpr (EPI == kEpiTwiddle) {
         const uint32_t m = (uint32_t)io.col * (uint32_t)i;  // col * i < big length <= 2^31
         const uint32_t mask = (1u << a.big_shift) - 1u;
@@ -419,7 +427,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
             }
         }
-        if constexpr (PRO == kProComplex || PRO == kProPlanar) {
+        if constexpr (PRO == kProComplex || PRO == kProPlanar || PRO == kProShift) {
             if (io.cin) {  // uniform: one scalar branch instead of a XOR per point
 #pragma unroll
                 for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
@@ -485,7 +493,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
             __syncthreads();
         }
     } else {
-        if constexpr (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiPlanar) {
+        if constexpr (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiPlanar || EPI == kEpiShift) {
             if (io.cout) {  // uniform branch
 #pragma unroll
                 for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
@@ -557,8 +565,22 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
     if (live) {
         io.col = t % a.cols;
         const int64_t o = t / a.cols;
-        const int64_t ib = io.col * a.in_inner + o * a.in_outer;
-        const int64_t ob = io.col * a.out_inner + o * a.out_outer;
+        int64_t oi = o;     // outer index read (input shift: rolled within groups of in_o_mod)
+        int64_t co = io.col;  // column index stored (output shift: rolled)
+        if constexpr (PRO == kProShift) {
+            if (a.in_o_mod > 1) {
+                const int64_t d = o % a.in_o_mod;
+                int64_t e = d - a.in_o_roll;
+                if (e < 0) e += a.in_o_mod;
+                oi = o - d + e;
+            }
+        }
+        if constexpr (EPI == kEpiShift) {
+            co += a.out_col_roll;
+            if (co >= a.cols) co -= a.cols;
+        }
+        const int64_t ib = io.col * a.in_inner + oi * a.in_outer;
+        const int64_t ob = co * a.out_inner + o * a.out_outer;
         if constexpr (PRO == kProReal) {
             io.in = reinterpret_cast<const float*>(a.in) + ib;
         } else if constexpr (PRO == kProPlanar) {
@@ -884,6 +906,9 @@ template <> struct VariantTraits<kVarColsSepOut> { static constexpr int map = kM
 template <> struct VariantTraits<kVarColsTwiddleSep> { static constexpr int map = kMapCols, pro = kProPlanar, epi = kEpiTwiddle; };
 template <> struct VariantTraits<kVarRowsTransposeSep> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiTransposePlanar; };
 template <> struct VariantTraits<kVarColsToRows> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiRowsOut; };
+template <> struct VariantTraits<kVarRowsShiftIn> { static constexpr int map = kMapRows, pro = kProShift, epi = kEpiComplex; };
+template <> struct VariantTraits<kVarRowsShiftBoth> { static constexpr int map = kMapRows, pro = kProShift, epi = kEpiShift; };
+template <> struct VariantTraits<kVarColsShiftOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiShift; };
 
 template <typename K>
 hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipStream_t stream)

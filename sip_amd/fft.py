@@ -24,10 +24,13 @@ _TYPE = {"c2c": N.ASCEND_FFT_C2C, "r2c": N.ASCEND_FFT_R2C, "c2r": N.ASCEND_FFT_C
 
 class Plan:
     """One plan handle.  dims: FFT sizes (1-3 of them); kind: 'c2c' | 'r2c' | 'c2r' | 'c2c_sep'
-    (C2C on separated real / imaginary float32 planes, run by execute_separated)."""
+    (C2C on separated real / imaginary float32 planes, run by execute_separated).  in_shift /
+    out_shift ('fft' | 'ifft') and scale fold asdFftShift / asdIfftShift permutations and an
+    output multiplier into a 1D or 2D C2C plan (asdFftExtSetShift)."""
 
     def __init__(self, kind: str, dims, batch: int, inverse: bool = False, vertical: bool = False,
-                 precision: str = "fp32"):
+                 precision: str = "fp32", in_shift: str | None = None, out_shift: str | None = None,
+                 scale: float = 1.0):
         self.L = N.lib()
         self.kind = kind
         self.dims = tuple(int(d) for d in dims)
@@ -43,6 +46,9 @@ class Plan:
             if precision != "fp32":
                 mode = {"split16": N.ASDFFT_PRECISION_SPLIT16}[precision]
                 N.check("asdFftExtSetPrecision", self.L.asdFftExtSetPrecision(h, mode))
+            if in_shift or out_shift or scale != 1.0:
+                modes = {None: N.ASDFFT_SHIFT_NONE, "fft": N.ASDFFT_SHIFT_FFT, "ifft": N.ASDFFT_SHIFT_IFFT}
+                N.check("asdFftExtSetShift", self.L.asdFftExtSetShift(h, modes[in_shift], modes[out_shift], scale))
             if len(self.dims) == 1:
                 st = self.L.asdFftMakePlan1D(h, self.dims[0], t, direction, self.batch,
                                              N.ASCEND_FFT_VERTICAL if vertical else N.ASCEND_FFT_HORIZONTAL)
