@@ -20,6 +20,7 @@ oracle's throughput on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -459,6 +460,59 @@ def bench_separated(torch, dev, stream, reps):
     return res
 
 
+def bench_shift(torch, dev, stream, reps):
+    """Shift form: y = fftshift(FFT2(ifftshift(x)) / (X*Y)) on a 2D C2C batch, as one plan with
+    the shifts folded in (asdFftExtSetShift) and as the reference's three calls (asdIfftShift
+    with the scale, exec, asdFftShift), event-timed through the C ABI on one stream.
+    Algorithmic bytes of the whole operation: x read once + y written once."""
+    from sip_amd import _native as N
+    from sip_amd.fft import Plan
+    L = N.lib()
+    res = {}
+    for X, Y, B in ((2048, 2048, 4), (1000, 1000, 16)):
+        g = torch.Generator(device=dev)
+        g.manual_seed(X + B)
+        x = torch.complex(torch.rand(B, X, Y, generator=g, device=dev) * 2 - 1,
+                          torch.rand(B, X, Y, generator=g, device=dev) * 2 - 1).contiguous()
+        y = torch.empty_like(x)
+        t1 = torch.empty_like(x)
+        t2 = torch.empty_like(x)
+        scale = 1.0 / (X * Y)
+        with torch.cuda.stream(stream):
+            fused = Plan("c2c", (X, Y), B, False, in_shift="ifft", out_shift="fft", scale=scale)
+            plain = Plan("c2c", (X, Y), B, False)
+            for p in (fused, plain):
+                p.bind_workspace(dev)
+                p.bind_stream(stream.cuda_stream)
+            tx, ty, tt1, tt2 = [N.AclTensor(t.shape, N.ACL_COMPLEX64, t.data_ptr()) for t in (x, y, t1, t2)]
+            sh = stream.cuda_stream
+
+            def run_fused():
+                if L.asdFftExecC2C(fused.handle, tx.ptr, ty.ptr) != 0:
+                    raise RuntimeError("fused exec failed")
+
+            def run_three():
+                if (L.asdIfftShift(X, Y, B, 0, scale, tx.ptr, tt1.ptr, ctypes.c_void_p(sh), None) != 0 or
+                        L.asdFftExecC2C(plain.handle, tt1.ptr, tt2.ptr) != 0 or
+                        L.asdFftShift(X, Y, B, 0, tt2.ptr, ty.ptr, ctypes.c_void_p(sh), None) != 0):
+                    raise RuntimeError("three-call sequence failed")
+            ms_f = event_time_ms(torch, stream, run_fused, reps)
+            yf = y.clone()
+            ms_3 = event_time_ms(torch, stream, run_three, reps)
+            torch.cuda.synchronize()
+            rel = (torch.linalg.vector_norm(yf - y) / torch.linalg.vector_norm(y)).item()
+            for t in (tx, ty, tt1, tt2):
+                t.close()
+            fused.close()
+            plain.close()
+        alg = 16.0 * x.numel()
+        res[f"shift_fft2_shift_{X}x{Y}_b{B}"] = {
+            "fused_us": round(ms_f * 1e3, 2), "three_calls_us": round(ms_3 * 1e3, 2),
+            "speedup": round(ms_3 / ms_f, 2), "fused_alg_GBps": round(alg / ms_f / 1e6, 1),
+            "fused_launches": 2, "three_call_launches": 4, "rel_l2_fused_vs_three": float(f"{rel:.3e}")}
+    return res
+
+
 def bench_istft(torch, dev, stream, reps):
     """ISTFT form: asdFftExecIstft on a complex64 (channel, nfft, nframes) spectrum (center,
     two-sided, complex output), event-timed through the C ABI, beside torch.istft (rocFFT) on
@@ -522,6 +576,8 @@ def main():
     ap.add_argument("--no-separated", action="store_true", help="skip the separated-planes form")
     ap.add_argument("--separated-only", action="store_true", help="run only the separated-planes form")
     ap.add_argument("--no-istft", action="store_true", help="skip the ISTFT form")
+    ap.add_argument("--no-shift", action="store_true", help="skip the folded-shift form")
+    ap.add_argument("--shift-only", action="store_true", help="run only the folded-shift form")
     ap.add_argument("--istft-only", action="store_true", help="run only the ISTFT form")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -546,6 +602,9 @@ def main():
         return
     if a.separated_only:
         print(json.dumps({"forms": {"separated": bench_separated(torch, dev, stream, max(5, a.kernel_reps))}}))
+        return
+    if a.shift_only:
+        print(json.dumps({"forms": {"shift": bench_shift(torch, dev, stream, max(5, a.kernel_reps))}}))
         return
     if a.istft_only:
         print(json.dumps({"forms": {"istft": bench_istft(torch, dev, stream, max(5, a.kernel_reps))}}))
@@ -698,6 +757,8 @@ def main():
         forms["separated"] = bench_separated(torch, dev, stream, max(5, a.kernel_reps))
     if not a.no_istft:
         forms["istft"] = bench_istft(torch, dev, stream, max(5, a.kernel_reps))
+    if not a.no_shift:
+        forms["shift"] = bench_shift(torch, dev, stream, max(5, a.kernel_reps))
     if forms:
         line["forms"] = forms
     if not a.no_cpu:

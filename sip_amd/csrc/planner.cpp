@@ -208,32 +208,41 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
         }
         if (found) break;
     }
-    // Long columns (2048) whose two LDS planes leave one workgroup per CU: a specialised kernel
-    // exchanging through one plane fits two workgroups of 8 columns per CU (measured 55 vs 70 us
-    // for the 2048 x 2048 x 4 column pass; 16 columns on one plane, one workgroup per CU, 62 us).
-    // ASDFFT_SPLIT_EMAX=0 disables it; ASDFFT_SPLIT_EMAX / ASDFFT_SPLIT_TPB pick the register
+    // Long columns whose two LDS planes leave one workgroup per CU: a specialised kernel
+    // exchanging through one plane takes the most columns (16 or 8) whose plane fits two
+    // workgroups per CU (measured on the 2048 x 2048 x 4 column pass: 55 us for 8 columns on one
+    // plane, 62 us for 16 on one plane with one workgroup per CU, 70 us for the two-plane kernel).
+    // ASDFFT_SPLIT_EMAX=0 disables it; ASDFFT_SPLIT_EMAX / ASDFFT_SPLIT_TPB force the register
     // budget / columns per workgroup of a specialised instance (measurement).
     const char* ns = getenv("ASDFFT_NO_STATIC");
     const char* se = getenv("ASDFFT_SPLIT_EMAX");
     const int split_only = se ? atoi(se) : -1;
     const char* st = getenv("ASDFFT_SPLIT_TPB");
-    const int split_tpb = st ? atoi(st) : 8;
-    if (variant == kVarColsC2C && found && c->tpb <= split_tpb && count >= split_tpb && split_only != 0 &&
-        c->lds > (size_t)n * (split_tpb + 1) * 4 && !(ns != nullptr && ns[0] == '1')) {
-        for (int emax : {32, 64}) {
-            if (split_only > 0 && emax != split_only) continue;
-            const int T = threads_for(n, radices, emax);
-            const size_t lds = (size_t)n * (split_tpb + 1) * 4;
-            if (T == 0 || split_tpb * T > 1024 || lds > kLdsLimit) continue;
-            if (find_static(kVarColsC2CSplit, emax, (int)n, T, split_tpb, radices.data(), (int)radices.size()) < 0)
-                continue;
-            c->emax = emax;
-            c->threads = T;
-            c->tpb = split_tpb;
-            c->plane = n * (split_tpb + 1);
-            c->lds = lds;
-            c->split = true;
-            break;
+    const int split_tpb_forced = st ? atoi(st) : 0;
+    const int split_variant = (variant == kVarColsC2C) ? (int)kVarColsC2CSplit
+                              : (variant == kVarColsShiftOut) ? (int)kVarColsShiftOutSplit : -1;
+    const size_t two_per_cu = kLdsLimit / 2;
+    if (split_variant >= 0 && found && c->lds > two_per_cu && split_only != 0 && !(ns != nullptr && ns[0] == '1')) {
+        bool done = false;
+        for (int tpb : {16, 8}) {
+            if (split_tpb_forced > 0 && tpb != split_tpb_forced) continue;
+            const size_t lds = (size_t)n * (tpb + 1) * 4;
+            if (count < tpb || (split_tpb_forced == 0 && lds > two_per_cu) || lds > kLdsLimit) continue;
+            for (int emax : {32, 64}) {
+                if (split_only > 0 && emax != split_only) continue;
+                const int T = threads_for(n, radices, emax);
+                if (T == 0 || tpb * T > 1024) continue;
+                if (find_static(split_variant, emax, (int)n, T, tpb, radices.data(), (int)radices.size()) < 0) continue;
+                c->emax = emax;
+                c->threads = T;
+                c->tpb = tpb;
+                c->plane = n * (tpb + 1);
+                c->lds = lds;
+                c->split = true;
+                done = true;
+                break;
+            }
+            if (done) break;
         }
     }
     return found;
@@ -309,7 +318,8 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     a.out_scale = 1.0f;
     fill_passes(c, &a, &offsets);
     st->direct = false;
-    st->shape.variant = c.split ? (int)kVarColsC2CSplit : variant;
+    st->shape.variant = !c.split ? variant : (variant == kVarColsShiftOut ? (int)kVarColsShiftOutSplit
+                                                                           : (int)kVarColsC2CSplit);
     st->shape.emax = c.emax;
     st->shape.block = c.tpb * c.threads;
     st->shape.lds_bytes = c.lds;

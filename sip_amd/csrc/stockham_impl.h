@@ -784,7 +784,7 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
 // points stay in registers across passes (the exchange overwrites them plane by plane:
 // all real parts are stored before any is read back, and the imaginary parts, still in
 // the previous pass's layout, are stored only after that).
-template <int EMAX, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
+template <int EMAX, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
 __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt, int tid, bool live, const Io& io,
                                              float2 (&v)[EMAX])
 {
@@ -834,7 +834,13 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
             const int j = tid + b * T;
             if (live && j < NR) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) out2[(j + r * NR) * s] = v[b * R + r];
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (EPI == kEpiShift) {
+                        store_output<kMapCols, kEpiShift>(a, io, N, j + r * NR, v[b * R + r]);
+                    } else {
+                        out2[(j + r * NR) * s] = v[b * R + r];
+                    }
+                }
             }
         }
     } else {
@@ -872,11 +878,11 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
             }
         });
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
-        split_passes<EMAX, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, lp, tt, tid, live, io, v);
+        split_passes<EMAX, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, lp, tt, tid, live, io, v);
     }
 }
 
-template <int EMAX, int N, int T, int TPB, int... Rs>
+template <int EMAX, int EPI, int N, int T, int TPB, int... Rs>
 __global__ void __launch_bounds__(T * TPB) stockham_split_kernel(const FftArgs a)
 {
     extern __shared__ float smem[];
@@ -885,9 +891,9 @@ __global__ void __launch_bounds__(T * TPB) stockham_split_kernel(const FftArgs a
     int64_t blk;
     bool live;
     Io io;
-    thread_setup<kMapCols, kProComplex, kEpiComplex>(a, g, tt, tid, blk, live, io);
+    thread_setup<kMapCols, kProComplex, EPI>(a, g, tt, tid, blk, live, io);
     float2 v[EMAX];
-    split_passes<EMAX, N, T, TPB, 0, 1, 0, Rs...>(a, smem, tt, tid, live, io, v);
+    split_passes<EMAX, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, smem, tt, tid, live, io, v);
 }
 
 // ------------------------------------------------------------------ launch helper
@@ -933,7 +939,9 @@ template <int V, int EMAX, int N, int T, int TPB, int... Rs>
 hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     if constexpr (V == kVarColsC2CSplit) {
-        return launch_kernel(stockham_split_kernel<EMAX, N, T, TPB, Rs...>, s, args, stream);
+        return launch_kernel(stockham_split_kernel<EMAX, kEpiComplex, N, T, TPB, Rs...>, s, args, stream);
+    } else if constexpr (V == kVarColsShiftOutSplit) {
+        return launch_kernel(stockham_split_kernel<EMAX, kEpiShift, N, T, TPB, Rs...>, s, args, stream);
     } else {
         using Tr = VariantTraits<V>;
         return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args,
