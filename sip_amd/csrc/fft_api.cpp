@@ -235,27 +235,7 @@ AspbStatus launch_steps(const asdfft::BuiltPlan& bp, char* const bufs[3], const 
             log_error(fn, "separated-plane step without its imaginary plane");
             return ACL_ERROR_INTERNAL_ERROR;
         }
-        hipError_t e;
-        if (s.mfma) {
-            asdfft::MfmaFftArgs a = s.margs;
-            a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
-            a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
-            e = asdfft::launch_mfma_fft1024(a, s.mfma_grid, stream);
-        } else if (s.direct) {
-            asdfft::DftArgs a = s.dargs;
-            a.in = bufs[s.src] + s.src_offset;
-            a.out = bufs[s.dst] + s.dst_offset;
-            a.in_im = s.planar_in ? in_im : nullptr;
-            a.out_im = s.planar_out ? out_im : nullptr;
-            e = asdfft::launch_dft_direct(s.shape, a, stream);
-        } else {
-            asdfft::FftArgs a = s.args;
-            a.in = bufs[s.src] + s.src_offset;
-            a.out = bufs[s.dst] + s.dst_offset;
-            a.in_im = s.planar_in ? in_im : nullptr;
-            a.out_im = s.planar_out ? out_im : nullptr;
-            e = asdfft::launch_stockham(s.shape, a, stream);
-        }
+        const hipError_t e = asdfft::launch_step(s, bufs, in_im, out_im, stream);
         if (e != hipSuccess) {
             log_error(fn, std::string("kernel launch failed: ") + hipGetErrorString(e));
             return ACL_ERROR_INTERNAL_ERROR;

@@ -133,6 +133,27 @@ struct DftArgs {
     int64_t in_cols, in_stride;
 };
 
+// Elementwise passes that turn a long complex transform into a real one (real_aux.hip).
+enum RealAuxOp : int {
+    kAuxR2CPost = 0,     // even R2C post-twiddle: Z[0..M) -> X[0..M]
+    kAuxC2RPre = 1,      // even C2R pre-twiddle: X[0..M] -> sequence of length M for a forward FFT
+    kAuxPromote = 2,     // odd R2C: real -> complex
+    kAuxHermExtend = 3,  // odd C2R: X[0..N/2] -> full Hermitian spectrum of length N
+    kAuxTruncate = 4,    // odd R2C: bins 0..N/2 of the full spectrum
+    kAuxRealPart = 5,    // odd C2R: real parts
+};
+
+struct RealAuxArgs {
+    const void* in;
+    void* out;
+    const float* tw;  // W_N^k, k in [0, N/2] (even ops)
+    int op;
+    int conj;         // conjugate the output (R2C post, truncate) or the input (C2R pre, extension)
+    int64_t n;        // real length N
+    int64_t count;    // transforms
+    int64_t in_dist, out_dist;  // per-transform distance in the op's input / output elements
+};
+
 // Overlap-add of inverse-transformed STFT frames (ISTFT): for output sample i of channel b,
 //   m = start + i,  y = sum_t frames[b][t][m - t*hop] * window[m - t*hop],
 //   env = sum_t window[m - t*hop]^2,  out[b][i] = y * scale / env

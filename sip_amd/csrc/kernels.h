@@ -64,6 +64,7 @@ extern const int kNumStaticInstances;
 int find_static(int variant, int emax, int n, int threads, int tpb, const int* radices, int nradices);
 hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStream_t stream);
 hipError_t launch_istft_ola(const OlaArgs& args, hipStream_t stream);
+hipError_t launch_real_aux(const RealAuxArgs& args, hipStream_t stream);
 
 }  // namespace asdfft
 

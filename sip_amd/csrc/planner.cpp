@@ -471,9 +471,14 @@ bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t str
 // Large complex length: n = n1 * n2, pass A = length-n2 columns (stride n1) with the
 // W_n^{n1 k2} twiddle folded into its store, pass B = length-n1 rows stored transposed.
 // planar: pass A reads and pass B writes a pair of float planes (real, imaginary).
+// conj_in / conj_out (-1: `inverse`) conjugate pass A's input and pass B's output; src_offset /
+// dst_offset are byte offsets into the source / destination buffers.
 bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size_t work_offset, BuiltPlan* bp,
-                  std::string* why, bool planar = false)
+                  std::string* why, bool planar = false, int conj_in = -1, int conj_out = -1, size_t src_offset = 0,
+                  size_t dst_offset = 0)
 {
+    if (conj_in < 0) conj_in = inverse ? 1 : 0;
+    if (conj_out < 0) conj_out = inverse ? 1 : 0;
     int64_t best1 = 0;
     double best_score = -1e30;
     for (int64_t n1 = 2; n1 <= 1024 && n1 < n; n1++) {
@@ -506,6 +511,7 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
 
     Step a;
     a.src = src;
+    a.src_offset = src_offset;
     a.dst = kBufWork;
     a.dst_offset = work_offset;
     a.planar_in = planar;
@@ -519,7 +525,7 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     ia.out_inner = 1;
     ia.out_outer = n;
     ia.out_stride = n1;
-    if (!stockham_step(n2, planar ? kVarColsTwiddleSep : kVarColsTwiddle, true, ia, inverse, 0, 0, &a, why)) return false;
+    if (!stockham_step(n2, planar ? kVarColsTwiddleSep : kVarColsTwiddle, true, ia, conj_in, 0, 0, &a, why)) return false;
     a.args.tw_big_lo = lo ? lo->dev : nullptr;
     a.args.tw_big_hi = hi ? hi->dev : nullptr;
     a.args.big_shift = shift;
@@ -532,6 +538,7 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     b.src = kBufWork;
     b.src_offset = work_offset;
     b.dst = dst;
+    b.dst_offset = dst_offset;
     b.planar_out = planar;
     b.bytes = 16.0 * (double)n * (double)count;
     StepIo ib;
@@ -545,8 +552,8 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
     ib.out_stride = n2;
     // transposed stores: 16 transforms per workgroup write 128-byte runs of complex
     // values, 32 those of a float plane
-    if (!stockham_step(n1, planar ? kVarRowsTransposeSep : kVarRowsTranspose, false, ib, 0, inverse, planar ? 32 : 16,
-                       &b, why))
+    if (!stockham_step(n1, planar ? kVarRowsTransposeSep : kVarRowsTranspose, false, ib, 0, conj_out,
+                       planar ? 32 : 16, &b, why))
         return false;
     b.describe += " (four-step B)";
     bp->steps.push_back(std::move(a));
@@ -604,6 +611,88 @@ bool real_rows(Kind kind, int64_t n, int64_t count, bool inverse, int src, int d
         }
     }
     bp->steps.push_back(std::move(st));
+    return true;
+}
+
+// True when a real length is beyond one workgroup but its complex transform has a four-step
+// schedule: even N with N/2 > 16384, odd N > 16384 (prime factors <= 19 either way).
+bool real_needs_large(int64_t n)
+{
+    const int64_t m = (n % 2 == 0) ? n / 2 : n;
+    return m > kMaxSingleKernel && !choose_radices(m).empty();
+}
+
+Step aux_step(int op, int64_t n, int64_t count, int conj, int src, size_t src_offset, int dst, size_t dst_offset,
+              int64_t in_dist, int64_t out_dist, double bytes)
+{
+    Step st;
+    st.aux = true;
+    st.src = src;
+    st.src_offset = src_offset;
+    st.dst = dst;
+    st.dst_offset = dst_offset;
+    st.bytes = bytes;
+    RealAuxArgs& a = st.raux;
+    a = RealAuxArgs{};
+    a.op = op;
+    a.conj = conj;
+    a.n = n;
+    a.count = count;
+    a.in_dist = in_dist;
+    a.out_dist = out_dist;
+    if (op == kAuxR2CPost || op == kAuxC2RPre) {
+        const int64_t m = n / 2;
+        TablePtr t = upload_table("aux:" + std::to_string(n), roots_table(n, m + 1));
+        if (t) st.tables.push_back(t);
+        st.tables_ready = (t != nullptr);
+        a.tw = t ? t->dev : nullptr;
+    }
+    static const char* names[] = {"r2c-post", "c2r-pre", "promote", "hermitian-extend", "truncate", "real-part"};
+    std::ostringstream os;
+    os << "real " << names[op] << " n=" << n << " count=" << count;
+    st.describe = os.str();
+    return st;
+}
+
+// Real transforms beyond one workgroup (1D): the complex four-step around elementwise passes.
+//   even R2C: four-step FFT_M of the input read as M complex -> post-twiddle into the output
+//   even C2R: pre-twiddle -> four-step FFT_M (conjugated output) written as the real output
+//   odd R2C:  promote to complex -> four-step FFT_N -> first N/2 + 1 bins
+//   odd C2R:  Hermitian extension -> four-step FFT_N -> real parts
+// Workspace: two complex buffers of the four-step's length per transform.
+bool real_large(Kind kind, int64_t n, int64_t count, bool inverse, BuiltPlan* bp, std::string* why)
+{
+    const bool even = (n % 2) == 0;
+    const int64_t m = even ? n / 2 : n;  // complex length of the four-step
+    const int64_t h = n / 2 + 1;
+    const size_t w = (size_t)(m * count * 8);
+    const double real_bytes = 4.0 * (double)n * (double)count, spec_bytes = 8.0 * (double)h * (double)count;
+    const double cplx_bytes = 8.0 * (double)m * (double)count;
+    bp->workspace = 2 * w;
+    if (kind == Kind::R2C) {
+        if (even) {
+            if (!c2c_fourstep(m, count, false, kBufIn, kBufWork, 0, bp, why, false, 0, 0, 0, w)) return false;
+            bp->steps.push_back(aux_step(kAuxR2CPost, n, count, inverse ? 1 : 0, kBufWork, w, kBufOut, 0, m, m + 1,
+                                         cplx_bytes + spec_bytes));
+        } else {
+            bp->steps.push_back(aux_step(kAuxPromote, n, count, 0, kBufIn, 0, kBufWork, 0, n, n, real_bytes + cplx_bytes));
+            if (!c2c_fourstep(m, count, false, kBufWork, kBufWork, w, bp, why, false, 0, 0, 0, 0)) return false;
+            bp->steps.push_back(aux_step(kAuxTruncate, n, count, inverse ? 1 : 0, kBufWork, 0, kBufOut, 0, n, h,
+                                         2.0 * spec_bytes));
+        }
+    } else {
+        if (even) {
+            bp->steps.push_back(aux_step(kAuxC2RPre, n, count, inverse ? 0 : 1, kBufIn, 0, kBufWork, 0, m + 1, m,
+                                         spec_bytes + cplx_bytes));
+            if (!c2c_fourstep(m, count, false, kBufWork, kBufOut, w, bp, why, false, 0, 1, 0, 0)) return false;
+        } else {
+            bp->steps.push_back(aux_step(kAuxHermExtend, n, count, inverse ? 1 : 0, kBufIn, 0, kBufWork, 0, h, n,
+                                         spec_bytes + cplx_bytes));
+            if (!c2c_fourstep(m, count, false, kBufWork, kBufWork, w, bp, why, false, 0, 0, 0, 0)) return false;
+            bp->steps.push_back(aux_step(kAuxRealPart, n, count, 0, kBufWork, 0, kBufOut, 0, n, n,
+                                         cplx_bytes + real_bytes));
+        }
+    }
     return true;
 }
 
@@ -850,6 +939,10 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
             if (!c2c_rows(n, B, inv, kBufIn, kBufOut, out, &why)) return why;
             return "";
         }
+        if (real_needs_large(n)) {
+            if (!real_large(spec.kind, n, B, inv, out, &why)) return why;
+            return "";
+        }
         if (!real_rows(spec.kind, n, B, inv, kBufIn, kBufOut, out, &why)) return why;
         return "";
     }
@@ -960,6 +1053,36 @@ std::string build_istft_frames(int64_t nfft, int64_t channel, int64_t nframes, B
     out->steps.push_back(std::move(st));
     out->workspace = (size_t)(count * nfft * 8);
     return "";
+}
+
+hipError_t launch_step(const Step& s, char* const bufs[3], const float* in_im, float* out_im, hipStream_t stream)
+{
+    if (s.mfma) {
+        MfmaFftArgs a = s.margs;
+        a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
+        a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
+        return launch_mfma_fft1024(a, s.mfma_grid, stream);
+    }
+    if (s.aux) {
+        RealAuxArgs a = s.raux;
+        a.in = bufs[s.src] + s.src_offset;
+        a.out = bufs[s.dst] + s.dst_offset;
+        return launch_real_aux(a, stream);
+    }
+    if (s.direct) {
+        DftArgs a = s.dargs;
+        a.in = bufs[s.src] + s.src_offset;
+        a.out = bufs[s.dst] + s.dst_offset;
+        a.in_im = s.planar_in ? in_im : nullptr;
+        a.out_im = s.planar_out ? out_im : nullptr;
+        return launch_dft_direct(s.shape, a, stream);
+    }
+    FftArgs a = s.args;
+    a.in = bufs[s.src] + s.src_offset;
+    a.out = bufs[s.dst] + s.dst_offset;
+    a.in_im = s.planar_in ? in_im : nullptr;
+    a.out_im = s.planar_out ? out_im : nullptr;
+    return launch_stockham(s.shape, a, stream);
 }
 
 std::string describe_plan(const BuiltPlan& plan)

@@ -65,6 +65,8 @@ enum class Precision {
 struct Step {
     bool direct = false;
     bool mfma = false;       // split-FP16 MFMA kernel (margs, mfma_grid)
+    bool aux = false;        // elementwise real pass (raux, real_aux.hip)
+    RealAuxArgs raux{};
     MfmaFftArgs margs{};
     int64_t mfma_grid = 0;
     LaunchShape shape{};
@@ -113,6 +115,10 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out);
 // b * nframes + t of a (channel, nframes, nfft) complex workspace (one launch, src kBufIn,
 // dst kBufWork).  Returns "" on success, otherwise the reason.
 std::string build_istft_frames(int64_t nfft, int64_t channel, int64_t nframes, BuiltPlan* out);
+
+// Launch of one step on `stream`: bufs = {input, output, workspace}; planar steps take the
+// imaginary planes from in_im / out_im.
+hipError_t launch_step(const Step& s, char* const bufs[3], const float* in_im, float* out_im, hipStream_t stream);
 
 // Human-readable description of the steps (ASDFFT_PLAN_TRACE=1 prints it).
 std::string describe_plan(const BuiltPlan& plan);

@@ -428,19 +428,7 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
         char* bufs[3] = {static_cast<char*>(const_cast<void*>(q->r.in)), static_cast<char*>(q->r.out),
                          static_cast<char*>(s.workspace)};
         for (const Step& st : p->built.steps) {
-            hipError_t e;
-            if (st.direct) {
-                DftArgs a = st.dargs;
-                a.in = bufs[st.src] + st.src_offset;
-                a.out = bufs[st.dst] + st.dst_offset;
-                e = launch_dft_direct(st.shape, a, s.stream);
-            } else {
-                FftArgs a = st.args;
-                a.in = bufs[st.src] + st.src_offset;
-                a.out = bufs[st.dst] + st.dst_offset;
-                e = launch_stockham(st.shape, a, s.stream);
-            }
-            if (e != hipSuccess) return 3;
+            if (launch_step(st, bufs, nullptr, nullptr, s.stream) != hipSuccess) return 3;
         }
         if (hipEventRecord(s.events[s.groups.size()], s.stream) != hipSuccess) return 3;
         s.groups.push_back({q->ticket});

@@ -326,3 +326,34 @@ def test_vertical_2048_columns(torch_dev, oracle):
     want = oracle.c2c_nd(np.ascontiguousarray(x.T), 1, False).T
     err = per_row_rel(np.ascontiguousarray(got.T), np.ascontiguousarray(want.T), 1).max()
     assert err <= tol_for((n,)), err
+
+
+# real lengths beyond one workgroup: even N with N/2 > 16384 (four-step on N/2 plus the
+# post/pre-twiddle pass), odd N > 16384 (four-step on N between promote / Hermitian-extension
+# and truncate / real-part passes)
+LARGE_REAL = [(65536, 2), (131072, 1), (40000, 3), (59049, 2), (16807, 2)]
+
+
+@pytest.mark.parametrize("n,batch", LARGE_REAL, ids=str)
+@pytest.mark.parametrize("inverse", [False, True])
+def test_large_real_r2c(torch_dev, oracle, n, batch, inverse):
+    rng = np.random.default_rng(n + batch)
+    x = rng.uniform(-1, 1, (batch, n)).astype(np.float32)
+    got = run_gpu(torch_dev, "r2c", (n,), batch, inverse, x)
+    want = oracle.r2c_nd(x, 1, inverse)
+    assert got.shape == want.shape
+    err = per_row_rel(got, want, 1).max()
+    assert err <= tol_for((n,)), (err, tol_for((n,)))
+
+
+@pytest.mark.parametrize("n,batch", LARGE_REAL, ids=str)
+@pytest.mark.parametrize("inverse", [False, True])
+def test_large_real_c2r(torch_dev, oracle, n, batch, inverse):
+    rng = np.random.default_rng(n + 7 * batch)
+    h = n // 2 + 1
+    x = (rng.uniform(-1, 1, (batch, h)) + 1j * rng.uniform(-1, 1, (batch, h))).astype(np.complex64)
+    got = run_gpu(torch_dev, "c2r", (n,), batch, inverse, x)
+    want = oracle.c2r_nd(x, (n,), inverse)
+    assert got.shape == want.shape
+    err = per_row_rel(got, want, 1).max()
+    assert err <= tol_for((n,)), (err, tol_for((n,)))

@@ -105,3 +105,17 @@ def test_every_smooth_length_up_to_16384_has_a_plan():
         assert "4000 entries" in p.describe()
     finally:
         p.close()
+
+
+def test_large_real_lengths_plan():
+    """Real lengths beyond one workgroup: even N (N/2 > 16384) run the four-step on N/2 with one
+    post- or pre-twiddle pass; odd N > 16384 run it on N between two elementwise passes."""
+    for kind, n, first, last in [("r2c", 65536, "four-step A", "r2c-post"), ("c2r", 65536, "c2r-pre", "four-step B"),
+                                 ("r2c", 131072, "four-step A", "r2c-post"), ("r2c", 59049, "promote", "truncate"),
+                                 ("c2r", 59049, "hermitian-extend", "real-part"), ("r2c", 40000, "four-step A", "r2c-post")]:
+        p = Plan(kind, (n,), 2)
+        s = steps(p.describe())
+        m = n // 2 if n % 2 == 0 else n
+        assert first in s[0] and last in s[-1], (kind, n, s)
+        assert p.workspace_size == 2 * m * 2 * 8, (kind, n, p.workspace_size)
+        p.close()
