@@ -184,7 +184,14 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
     }
     // 16 adjacent complex columns are 128-byte runs; a float plane needs 32 for the same
     // (a half-line read by two workgroups on different XCDs is fetched twice)
-    const int first_tpb = (variant == kVarColsTwiddleSep) ? 32 : 16;
+    // ISTFT frames (kVarColsToRows) read columns nframes * 8 bytes apart, a stride that is in
+    // general not a multiple of a line: runs of 4 columns and more workgroups per CU measured
+    // fastest (1024 x 16000 columns: 119.7 us at 4 per workgroup, 134.3 at 8, 150.9 at 16;
+    // 400 x 32000: 93.4 / 117.6 / 127.7 us)
+    const int first_tpb = (variant == kVarColsTwiddleSep) ? 32 : (variant == kVarColsToRows) ? 4 : 16;
+    // ASDFFT_COLS_LDS_CAP=<bytes> caps a column workgroup's LDS (measurement)
+    const char* capenv = getenv("ASDFFT_COLS_LDS_CAP");
+    const size_t lds_cap = capenv ? (size_t)atol(capenv) : kLdsLimit;
     bool found = false;
     for (int tpb : {32, 16, 8, 4, 2, 1}) {
         if (tpb > first_tpb) continue;
@@ -195,7 +202,7 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
             const int64_t plane = n * (tpb + 1);  // column layout: row pitch tpb + 1 words
             const bool need_lds = radices.size() > 1 || variant == kVarColsToRows;  // (epilogue via LDS)
             const size_t lds = need_lds ? (size_t)(8 * plane) : 0;
-            if (lds > kLdsLimit) continue;
+            if (lds > kLdsLimit || (tpb > 1 && lds > lds_cap)) continue;
             c->radices = radices;
             c->emax = emax;
             c->threads = T;
