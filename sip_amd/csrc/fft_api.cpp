@@ -1208,9 +1208,9 @@ extern "C" int32_t asdFftExtSetShift(void* handle, int32_t in_shift, int32_t out
         log_error(fn, "unknown shift mode");
         return ACL_ERROR_INVALID_PARAM;
     }
-    if (!(scale == scale) || scale == std::numeric_limits<double>::infinity() ||
-        scale == -std::numeric_limits<double>::infinity()) {
-        log_error(fn, "scale must be finite");
+    const double fmax = (double)std::numeric_limits<float>::max();
+    if (!(scale >= -fmax && scale <= fmax)) {  // NaN fails both comparisons
+        log_error(fn, "scale must be a finite float32 value");
         return ACL_ERROR_INVALID_PARAM;
     }
     sp->in_shift = in_shift == ASDFFT_SHIFT_FFT ? asdfft::kShiftFft

@@ -46,6 +46,7 @@ def test_set_shift_rules(L):
         assert L.asdFftExtSetShift(h, S0, -1, 1.0) == INVALID
         assert L.asdFftExtSetShift(h, SI, SF, float("nan")) == INVALID
         assert L.asdFftExtSetShift(h, SI, SF, float("inf")) == INVALID
+        assert L.asdFftExtSetShift(h, SI, SF, 1e300) == INVALID  # overflows float32
         assert L.asdFftExtSetShift(h, SI, SF, 0.25) == OK
         assert L.asdFftMakePlan2D(h, 64, 48, C2C, FWD, 2) == OK
         assert L.asdFftExtSetShift(h, SF, SF, 1.0) == INVALID  # the plan exists
