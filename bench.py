@@ -128,10 +128,17 @@ def make_inputs(torch, dev, kind, dims, batch, seed):
     raise ValueError(kind)
 
 
-# per-launch timing rotates over at least this many bytes of (input, output) pairs: more than
-# twice the MI355X's 256 MB device cache (MALL)
+# per-launch timing rotates over copies of the (input, output) pair such that each of the two
+# alone spans at least this many bytes, more than twice the MI355X's 256 MB device cache (MALL):
+# an in-place step touches only the output, so sizing by the pair would leave its working set
+# in the cache
 RING_BYTES = 512 << 20
 RING_MAX = 256
+
+
+def ring_copies(x, y):
+    smaller = min(x.numel() * x.element_size(), y.numel() * y.element_size())
+    return max(1, min(RING_MAX, -(-RING_BYTES // smaller)))
 
 
 def event_time_ms(torch, stream, fn, reps):
@@ -178,6 +185,93 @@ def cpu_baseline(seconds_budget=20.0):
             "sample": "FP64 C oracle, 1 thread: 8 transforms each of C2C N=1000/2187/6000/4096, R2C and C2R "
                       "N=8192, plus one 2048x2048 2D and one 256^3 3D transform; "
                       f"{total_flop / 1e9:.2f} GFLOP in {dt:.1f} s"}
+
+
+def cpu_numpy(max_threads=16):
+    """numpy.fft on the host cores over one full step of the coverage workload, two ways:
+    golden: as the reference's unit-test golden generators call it, on one thread: one
+      np.fft.fft / rfft / irfft(norm="forward") per row, cast to complex64
+      (tests/ut/unittest/fft/1d/c2c/c2c_data/gen_data.py:42-65, r2c_data/gen_data.py:43,
+      c2r_data/gen_data.py:48), and one np.fft.fft2 / fftn over the whole 2D / 3D array
+      (2d/c2c/c2c_2d_data/gen_data_2d.py:65, 3d/c2c/c2c_3d_data/gen_data_3d.py:44);
+    threads: the same transforms as batched numpy calls over chunks on a thread pool (numpy's
+      FFT releases the GIL), one axis at a time for 2D / 3D, each split along the largest other
+      axis."""
+    import concurrent.futures as cf
+
+    import numpy as np
+    rng = np.random.default_rng(0)
+    threads = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    data = []
+    for name, kind, dims, batch, inv, _ in WORKLOAD:
+        shape = (batch,) + dims
+        if kind == "c2c":
+            x = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+        elif kind == "r2c":
+            x = rng.standard_normal(shape).astype(np.float32)
+        else:
+            x = np.fft.rfft(rng.standard_normal(shape)).astype(np.complex64)
+        data.append((kind, dims, batch, x))
+
+    def golden():
+        for kind, dims, batch, x in data:
+            n = dims[-1]
+            if len(dims) == 2:
+                np.fft.fft2(x, s=dims).astype(np.complex64)
+            elif len(dims) == 3:
+                np.fft.fftn(x, axes=(-3, -2, -1)).astype(np.complex64)
+            elif kind == "c2c":
+                for b in range(batch):
+                    np.fft.fft(x[b], n=n).astype(np.complex64)
+            elif kind == "r2c":
+                for b in range(batch):
+                    np.fft.rfft(x[b], n=n).astype(np.complex64)
+            else:
+                for b in range(batch):
+                    np.fft.irfft(x[b], n=n, norm="forward").astype(np.float32)
+
+    def pooled(ex):
+        for kind, dims, batch, x in data:
+            n = dims[-1]
+            if kind != "c2c":
+                f = (lambda a: np.fft.rfft(a, axis=-1).astype(np.complex64)) if kind == "r2c" else \
+                    (lambda a: np.fft.irfft(a, n=n, axis=-1, norm="forward").astype(np.float32))
+                list(ex.map(f, np.array_split(x, threads * 4)))
+                continue
+            y = x.copy()
+            for ax in range(y.ndim - 1, 0, -1):
+                split = max((d for d in range(y.ndim) if d != ax), key=lambda d: y.shape[d])
+                bounds = np.linspace(0, y.shape[split], min(y.shape[split], 4 * threads) + 1).astype(int)
+
+                def work(lo_hi, ax=ax, split=split):
+                    idx = [slice(None)] * y.ndim
+                    idx[split] = slice(*lo_hi)
+                    y[tuple(idx)] = np.fft.fft(y[tuple(idx)], axis=ax)
+                list(ex.map(work, zip(bounds[:-1], bounds[1:])))
+
+    total_flop = sum(flops(dims, batch) for _, dims, batch, _ in data)
+    t0 = time.perf_counter()
+    golden()
+    t_golden = time.perf_counter() - t0
+    with cf.ThreadPoolExecutor(threads) as ex:
+        pooled(ex)  # untimed: thread start-up
+        t0 = time.perf_counter()
+        pooled(ex)
+        t_pool = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"golden": {"gflops": round(total_flop / t_golden / 1e9, 3), "s_per_step": round(t_golden, 3),
+                       "threads": 1},
+            "threads": {"gflops": round(total_flop / t_pool / 1e9, 3), "s_per_step": round(t_pool, 3),
+                        "threads": threads},
+            "cpu_model": model, "cpus_visible": os.cpu_count(), "numpy": np.__version__,
+            "sample": "one full step of the coverage workload (same shapes and batches), seeded numpy inputs"}
 
 
 def bench_ragged(torch, dev, stream, reps, warmup):
@@ -460,6 +554,102 @@ def bench_separated(torch, dev, stream, reps):
     return res
 
 
+# (name, kind, dims, batch, inverse): every configuration of the plain FP32 drop-in, the
+# C2C N=4096 batch 512 case and the two long/short C2C shapes of the precision form
+VENDOR_CASES = [
+    ("c2c_n4096_b512", "c2c", (4096,), 512, False),
+    ("c2c_n1000_b256", "c2c", (1000,), 256, False),
+    ("c2c_n2187_b256", "c2c", (2187,), 256, False),
+    ("c2c_n6000_b256", "c2c", (6000,), 256, False),
+    ("c2c_n4096_b256", "c2c", (4096,), 256, False),
+    ("r2c_n8192_b1024", "r2c", (8192,), 1024, False),
+    ("c2r_n8192_b1024", "c2r", (8192,), 1024, True),
+    ("c2c2d_2048x2048_b4", "c2c", (2048, 2048), 4, False),
+    ("c2c3d_256x256x256_b1", "c2c", (256, 256, 256), 1, False),
+    ("c2c_n65536_b64", "c2c", (65536,), 64, False),
+    ("c2c_n1024_b8192", "c2c", (1024,), 8192, False),
+]
+
+
+def bench_vendor(torch, dev, stream, reps):
+    """Vendor comparator: the same transforms through torch.fft, which runs hipFFT/rocFFT on
+    ROCm, beside this library's C-ABI exec.  Both are event-timed on one stream over the same
+    ring of inputs (ring_copies: each side spans at least RING_BYTES, so the device cache
+    cannot serve a repetition).  The
+    C-ABI calls write to the ring's outputs; torch.fft allocates each output, and the previous
+    output of the same ring slot is released just before, so its writes rotate over as many
+    blocks.  torch's C2R clones its input first (rocFFT's C2R overwrites its input): that copy
+    is part of the torch figure.  agree = relative L2 between the two outputs."""
+    from sip_amd import _native as N
+    from sip_amd.fft import Plan
+    L = N.lib()
+    res = {}
+    for name, kind, dims, batch, inv in VENDOR_CASES:
+        rank = len(dims)
+        with torch.cuda.stream(stream):
+            plan = Plan(kind, dims, batch, inv)
+            plan.bind_workspace(dev)
+            plan.bind_stream(stream.cuda_stream)
+            if kind == "c2r":
+                n = dims[-1]
+                g = torch.Generator(device=dev)
+                g.manual_seed(5)
+                x0 = torch.fft.rfft(torch.randn((batch, n), generator=g, device=dev)).contiguous()
+            else:
+                x0 = make_inputs(torch, dev, kind, dims, batch, len(name))
+            out_dtype = torch.float32 if kind == "c2r" else torch.complex64
+            y0 = torch.empty(plan.out_shape(), dtype=out_dtype, device=dev)
+            copies = ring_copies(x0, y0)
+            xs = [x0] + [x0.clone() for _ in range(copies - 1)]
+            ys = [y0] + [torch.empty_like(y0) for _ in range(copies - 1)]
+            acl = lambda t: N.AclTensor(t.shape, N.ACL_FLOAT if t.dtype == torch.float32 else N.ACL_COMPLEX64,
+                                        t.data_ptr())
+            tins, touts = [acl(t) for t in xs], [acl(t) for t in ys]
+            fn = {"c2c": L.asdFftExecC2C, "r2c": L.asdFftExecR2C, "c2r": L.asdFftExecC2R}[kind]
+            if kind == "c2r":
+                tf = lambda x: torch.fft.irfft(x, n=dims[-1], norm="forward")
+            elif kind == "r2c":
+                tf = torch.fft.rfft
+            elif rank == 1:
+                tf = torch.fft.fft
+            else:
+                tf = lambda x: torch.fft.fftn(x, dim=tuple(range(-rank, 0)))
+            torch.cuda.synchronize()
+            cur = [0]
+
+            def ours():
+                i = cur[0] = (cur[0] + 1) % copies
+                st = fn(plan.handle, tins[i].ptr, touts[i].ptr)
+                if st != 0:
+                    raise RuntimeError(f"{name}: exec returned {st}")
+            ms_ours = event_time_ms(torch, stream, ours, reps)
+            outs = [None] * copies
+
+            def vendor():
+                i = cur[0] = (cur[0] + 1) % copies
+                outs[i] = None
+                outs[i] = tf(xs[i])
+            ms_vendor = event_time_ms(torch, stream, vendor, reps)
+            # agreement on one ring slot: both outputs of the same input
+            ours()
+            i = cur[0]
+            ref = tf(xs[i])
+            torch.cuda.synchronize()
+            agree = (torch.linalg.vector_norm((ys[i] - ref).flatten()) /
+                     torch.linalg.vector_norm(ref.flatten())).item()
+            for t in tins + touts:
+                t.close()
+            plan.close()
+            del xs, ys, outs, ref
+        res[name] = {"ours_us": round(ms_ours * 1e3, 2), "rocfft_us": round(ms_vendor * 1e3, 2),
+                     "speedup": round(ms_vendor / ms_ours, 3),
+                     "ours_hbm_frac": round(alg_bytes(kind, dims, batch) / ms_ours / 1e6 / HBM_PEAK_GBS, 4),
+                     "rocfft_hbm_frac": round(alg_bytes(kind, dims, batch) / ms_vendor / 1e6 / HBM_PEAK_GBS, 4),
+                     "agree_rel_l2": float(f"{agree:.3g}"), "ring_pairs": copies}
+    torch.cuda.empty_cache()
+    return res
+
+
 def bench_shift(torch, dev, stream, reps):
     """Shift form: y = fftshift(FFT2(ifftshift(x)) / (X*Y)) on a 2D C2C batch, as one plan with
     the shifts folded in (asdFftExtSetShift) and as the reference's three calls (asdIfftShift
@@ -579,6 +769,8 @@ def main():
     ap.add_argument("--no-shift", action="store_true", help="skip the folded-shift form")
     ap.add_argument("--shift-only", action="store_true", help="run only the folded-shift form")
     ap.add_argument("--istft-only", action="store_true", help="run only the ISTFT form")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocFFT (torch.fft) comparator form")
+    ap.add_argument("--vendor-only", action="store_true", help="run only the rocFFT (torch.fft) comparator form")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
@@ -608,6 +800,9 @@ def main():
         return
     if a.istft_only:
         print(json.dumps({"forms": {"istft": bench_istft(torch, dev, stream, max(5, a.kernel_reps))}}))
+        return
+    if a.vendor_only:
+        print(json.dumps({"forms": {"vendor": bench_vendor(torch, dev, stream, max(5, a.kernel_reps))}}))
         return
     L = N.lib()
     entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
@@ -653,13 +848,13 @@ def main():
         wall_ms = (time.perf_counter() - t_wall0) * 1e3 / a.steps
 
         # per-workload and per-launch timings (same stream, same kernels).  Each repetition
-        # takes the next (input, output) pair of a ring of at least RING_BYTES, so the 256 MB
-        # device cache (MALL) cannot serve a repetition from the previous one's data.
+        # takes the next (input, output) pair of a ring whose inputs and whose outputs each span
+        # at least RING_BYTES, so the 256 MB device cache (MALL) cannot serve a repetition from
+        # an earlier one's data, in-place steps included.
         breakdown = []
         launches = []
         for j in jobs:
-            pair = j["x"].numel() * j["x"].element_size() + j["y"].numel() * j["y"].element_size()
-            copies = max(1, min(RING_MAX, -(-RING_BYTES // pair)))
+            copies = ring_copies(j["x"], j["y"])
             ring = [(j["tin"], j["tout"])]
             keep = []
             for _ in range(copies - 1):
@@ -759,10 +954,14 @@ def main():
         forms["istft"] = bench_istft(torch, dev, stream, max(5, a.kernel_reps))
     if not a.no_shift:
         forms["shift"] = bench_shift(torch, dev, stream, max(5, a.kernel_reps))
+    if not a.no_vendor:
+        forms["vendor"] = bench_vendor(torch, dev, stream, max(5, a.kernel_reps))
     if forms:
         line["forms"] = forms
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
+        forms["cpu_numpy"] = cpu_numpy()
+        line["forms"] = forms
     for j in jobs:
         j["tin"].close()
         j["tout"].close()

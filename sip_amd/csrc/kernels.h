@@ -32,6 +32,7 @@ enum Variant : int {
     kVarRowsShiftBoth = 17,     // rows, input and output shift
     kVarColsShiftOut = 18,      // columns, output shift
     kVarColsShiftOutSplit = 19, // kVarColsShiftOut exchanging through one LDS plane (specialised only)
+    kVarRowsC2CSplit = 20,      // rows C2C exchanging through one LDS plane (specialised only)
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a

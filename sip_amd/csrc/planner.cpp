@@ -102,7 +102,7 @@ struct Config {
     int64_t tstride = 0;  // rows mapping
     int64_t plane = 0;
     size_t lds = 0;
-    bool split = false;  // one-plane LDS exchange (kVarColsC2CSplit)
+    bool split = false;  // one-plane LDS exchange (the variant's specialised split kernel)
 };
 
 // Rows mapping: lanes along a transform; several short transforms per workgroup.
@@ -157,8 +157,32 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
         found = true;
         if (want_tpb <= 0) break;  // first feasible register budget (EMAX 16 preferred)
     }
+    // ASDFFT_ROWS_SPLIT=1: rows C2C on a specialised one-plane instance for the chosen geometry
+    // (half the LDS, twice the resident workgroups).  Opt-in: measured no faster than the
+    // two-plane kernel (1024 x 8192: 28.6 vs 28.7 us; 1000 / 2187 / 6000 x 256: 9-10% slower),
+    // the points carried in registers across passes cost more than the occupancy gains.
+    const char* rs = getenv("ASDFFT_ROWS_SPLIT");
+    const char* ns = getenv("ASDFFT_NO_STATIC");
+    if (found && variant == kVarRowsC2C && best.lds > 0 && rs != nullptr && rs[0] == '1' &&
+        !(ns != nullptr && ns[0] == '1') &&
+        find_static(kVarRowsC2CSplit, best.emax, (int)n, best.threads, best.tpb, radices.data(),
+                    (int)radices.size()) >= 0) {
+        best.split = true;
+        best.lds = (size_t)(4 * best.tpb * best.tstride);
+    }
     if (found) *c = best;
     return found;
+}
+
+// The one-plane specialised variant of a two-plane variant (-1: none).
+int split_variant_of(int variant)
+{
+    switch (variant) {
+        case kVarColsC2C: return kVarColsC2CSplit;
+        case kVarColsShiftOut: return kVarColsShiftOutSplit;
+        case kVarRowsC2C: return kVarRowsC2CSplit;
+        default: return -1;
+    }
 }
 
 // Column mapping: lanes across adjacent transforms (columns) for coalesced strided access.
@@ -216,25 +240,26 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
         if (found) break;
     }
     // Long columns whose two LDS planes leave one workgroup per CU: a specialised kernel
-    // exchanging through one plane takes the most columns (16 or 8) whose plane fits two
-    // workgroups per CU (measured on the 2048 x 2048 x 4 column pass: 55 us for 8 columns on one
-    // plane, 62 us for 16 on one plane with one workgroup per CU, 70 us for the two-plane kernel).
-    // ASDFFT_SPLIT_EMAX=0 disables it; ASDFFT_SPLIT_EMAX / ASDFFT_SPLIT_TPB force the register
-    // budget / columns per workgroup of a specialised instance (measurement).
+    // exchanging through one plane takes the most columns (16, else 8) whose plane fits the
+    // LDS.  Full 128-byte runs matter more than a second workgroup per CU once the data comes
+    // from HBM rather than the device cache: the 2048 x 2048 x 4 column pass, timed over a ring
+    // of outputs larger than that cache, takes 74.5 us with 16 columns on one plane (139 KiB,
+    // one workgroup per CU), 83.3 us with 8 (72 KiB, two per CU) and 95.3 us on the two-plane
+    // kernel (8 columns).  ASDFFT_SPLIT_EMAX=0 disables it; ASDFFT_SPLIT_EMAX / ASDFFT_SPLIT_TPB
+    // force the register budget / columns per workgroup of a specialised instance (measurement).
     const char* ns = getenv("ASDFFT_NO_STATIC");
     const char* se = getenv("ASDFFT_SPLIT_EMAX");
     const int split_only = se ? atoi(se) : -1;
     const char* st = getenv("ASDFFT_SPLIT_TPB");
     const int split_tpb_forced = st ? atoi(st) : 0;
-    const int split_variant = (variant == kVarColsC2C) ? (int)kVarColsC2CSplit
-                              : (variant == kVarColsShiftOut) ? (int)kVarColsShiftOutSplit : -1;
+    const int split_variant = split_variant_of(variant);
     const size_t two_per_cu = kLdsLimit / 2;
     if (split_variant >= 0 && found && c->lds > two_per_cu && split_only != 0 && !(ns != nullptr && ns[0] == '1')) {
         bool done = false;
         for (int tpb : {16, 8}) {
             if (split_tpb_forced > 0 && tpb != split_tpb_forced) continue;
             const size_t lds = (size_t)n * (tpb + 1) * 4;
-            if (count < tpb || (split_tpb_forced == 0 && lds > two_per_cu) || lds > kLdsLimit) continue;
+            if (count < tpb || lds > kLdsLimit) continue;
             for (int emax : {32, 64}) {
                 if (split_only > 0 && emax != split_only) continue;
                 const int T = threads_for(n, radices, emax);
@@ -325,8 +350,7 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     a.out_scale = 1.0f;
     fill_passes(c, &a, &offsets);
     st->direct = false;
-    st->shape.variant = !c.split ? variant : (variant == kVarColsShiftOut ? (int)kVarColsShiftOutSplit
-                                                                           : (int)kVarColsC2CSplit);
+    st->shape.variant = !c.split ? variant : split_variant_of(variant);
     st->shape.emax = c.emax;
     st->shape.block = c.tpb * c.threads;
     st->shape.lds_bytes = c.lds;

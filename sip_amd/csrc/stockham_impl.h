@@ -775,27 +775,30 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
     lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
 }
 
-// ------------------------------------------------------------------ one-plane exchange (columns)
-// Specialised column kernel whose passes exchange through ONE LDS plane: first the real
-// parts of every point, then the imaginary parts.  It needs half the LDS of the two-plane
-// exchange, so for long columns (2048 points, 8 columns: 72 KiB instead of 144 KiB) two
-// workgroups share a CU and one's loads overlap the other's passes; the cost is two more
-// barriers per exchange.  The
-// points stay in registers across passes (the exchange overwrites them plane by plane:
-// all real parts are stored before any is read back, and the imaginary parts, still in
-// the previous pass's layout, are stored only after that).
-template <int EMAX, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
-__device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt, int tid, bool live, const Io& io,
-                                             float2 (&v)[EMAX])
+// ------------------------------------------------------------------ one-plane exchange
+// Specialised kernel whose passes exchange through ONE LDS plane: first the real parts of
+// every point, then the imaginary parts.  It needs half the LDS of the two-plane exchange,
+// so twice the workgroups fit a CU and one's loads overlap the others' passes; the cost is
+// two more barriers per exchange.  Columns: long columns (2048 points, 8 columns: 72 KiB
+// instead of 144 KiB) run two workgroups per CU instead of one.  Rows: 256-thread
+// workgroups of EMAX-16 transforms (about 17 KiB instead of 34) reach the 8 waves per SIMD
+// that their register use allows instead of 4.  The points stay in registers across passes
+// (the exchange overwrites them plane by plane: all real parts are stored before any is
+// read back, and the imaginary parts, still in the previous pass's layout, are stored only
+// after that).
+template <int EMAX, int MAP, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
+__device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, float* lp, int tt, int tid, bool live,
+                                             const Io& io, float2 (&v)[EMAX])
 {
     constexpr int B = EMAX / R;
     constexpr int NR = N / R;
     constexpr int PITCH = TPB + 1;  // column layout: slot of element i of column tt = i * PITCH + tt
+    constexpr int kTstride = N + (N >> 4) + 1;  // rows layout: one padded block per transform
     static_assert(B >= 1, "radix larger than the per-thread register budget");
-    float* lp_t = lp + tt;
+    float* lp_t = (MAP == kMapRows) ? lp + tt * kTstride : lp + tt;
     if constexpr (P == 0) {
         const float2* in2 = reinterpret_cast<const float2*>(io.in);
-        const int s = io.in_stride;
+        const int s = stride_of<MAP>(io.in_stride);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
@@ -828,7 +831,7 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
             for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
         }
         float2* out2 = reinterpret_cast<float2*>(io.out);
-        const int s = io.out_stride;
+        const int s = stride_of<MAP>(io.out_stride);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
@@ -836,7 +839,7 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (EPI == kEpiShift) {
-                        store_output<kMapCols, kEpiShift>(a, io, N, j + r * NR, v[b * R + r]);
+                        store_output<MAP, kEpiShift>(a, io, N, j + r * NR, v[b * R + r]);
                     } else {
                         out2[(j + r * NR) * s] = v[b * R + r];
                     }
@@ -856,8 +859,14 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
                 if (j < NR) {
                     const int k = (SPAN > 1) ? j % SPAN : 0;
                     const int base = (j - k) * R + k;
+                    const int pb = lds_pbase<MAP>(g, base);
+                    constexpr bool kBase16 = (SPAN == 1) && (R % 16 == 0);  // base = j * R
 #pragma unroll
-                    for (int r = 0; r < R; ++r) lp_t[(base + r * SPAN) * PITCH] = kIm ? v[b * R + r].y : v[b * R + r].x;
+                    for (int r = 0; r < R; ++r) {
+                        const int at = (MAP == kMapRows) ? lds_rel<kMapRows, false>(g, base, pb, r * SPAN, kBase16)
+                                                         : (base + r * SPAN) * PITCH;
+                        lp_t[at] = kIm ? v[b * R + r].y : v[b * R + r].x;
+                    }
                 }
             }
             __syncthreads();
@@ -865,9 +874,12 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
             for (int b = 0; b < BN; ++b) {
                 const int j = tid + b * T;
                 if (j < NRN) {
+                    const int pj = lds_pbase<MAP>(g, j);
 #pragma unroll
                     for (int r = 0; r < RN; ++r) {
-                        const float x = lp_t[(j + r * NRN) * PITCH];
+                        const int at = (MAP == kMapRows) ? lds_rel<kMapRows, false>(g, j, pj, r * NRN, false)
+                                                         : (j + r * NRN) * PITCH;
+                        const float x = lp_t[at];
                         if constexpr (kIm) {
                             v[b * RN + r].y = x;
                         } else {
@@ -878,22 +890,24 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, float* lp, int tt
             }
         });
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
-        split_passes<EMAX, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, lp, tt, tid, live, io, v);
+        split_passes<EMAX, MAP, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lp, tt, tid, live, io, v);
     }
 }
 
-template <int EMAX, int EPI, int N, int T, int TPB, int... Rs>
-__global__ void __launch_bounds__(T * TPB) stockham_split_kernel(const FftArgs a)
+// Rows mapping keeps transforms per workgroup at run time (the launch bound is the
+// register-budget block); columns take it as a constant.
+template <int EMAX, int MAP, int EPI, int N, int T, int TPB, int... Rs>
+__global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) stockham_split_kernel(const FftArgs a)
 {
     extern __shared__ float smem[];
-    const Geo g{N, T, TPB, 0};
+    const Geo g{N, T, (MAP == kMapRows) ? a.tpb : TPB, N + (N >> 4) + 1};
     int tt, tid;
     int64_t blk;
     bool live;
     Io io;
-    thread_setup<kMapCols, kProComplex, EPI>(a, g, tt, tid, blk, live, io);
+    thread_setup<MAP, kProComplex, EPI>(a, g, tt, tid, blk, live, io);
     float2 v[EMAX];
-    split_passes<EMAX, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, smem, tt, tid, live, io, v);
+    split_passes<EMAX, MAP, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, smem, tt, tid, live, io, v);
 }
 
 // ------------------------------------------------------------------ launch helper
@@ -939,9 +953,11 @@ template <int V, int EMAX, int N, int T, int TPB, int... Rs>
 hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     if constexpr (V == kVarColsC2CSplit) {
-        return launch_kernel(stockham_split_kernel<EMAX, kEpiComplex, N, T, TPB, Rs...>, s, args, stream);
+        return launch_kernel(stockham_split_kernel<EMAX, kMapCols, kEpiComplex, N, T, TPB, Rs...>, s, args, stream);
     } else if constexpr (V == kVarColsShiftOutSplit) {
-        return launch_kernel(stockham_split_kernel<EMAX, kEpiShift, N, T, TPB, Rs...>, s, args, stream);
+        return launch_kernel(stockham_split_kernel<EMAX, kMapCols, kEpiShift, N, T, TPB, Rs...>, s, args, stream);
+    } else if constexpr (V == kVarRowsC2CSplit) {
+        return launch_kernel(stockham_split_kernel<EMAX, kMapRows, kEpiComplex, N, T, TPB, Rs...>, s, args, stream);
     } else {
         using Tr = VariantTraits<V>;
         return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args,

@@ -292,8 +292,8 @@ def test_specialised_kernels_match_runtime_kernel(torch_dev, oracle, monkeypatch
     assert checked >= 20
 
 
-# long column passes (the one-plane LDS exchange kernel: 2048 points, 8 columns per
-# workgroup; 1000 points, 16 columns): full and partial workgroups, 2D, 3D and VERTICAL 1D
+# long column passes (the one-plane LDS exchange kernel: 2048 and 1000 points, 16 columns
+# per workgroup): full and partial workgroups, 2D, 3D and VERTICAL 1D
 LONG_COLUMNS = [("c2c", (2048, 64), 1, False), ("c2c", (2048, 40), 2, True), ("c2c", (2048, 4, 8), 1, False),
                 ("c2c", (2048, 24), 1, True), ("c2c", (1000, 40), 2, False), ("c2c", (1000, 1000), 1, True)]
 
@@ -307,6 +307,40 @@ def test_long_column_passes(torch_dev, oracle, kind, dims, batch, inverse):
     want = oracle.c2c_nd(x, len(dims), inverse)
     err = per_row_rel(got, want, len(dims)).max()
     assert err <= tol_for(dims), (err, tol_for(dims))
+
+
+# one-plane geometries that plans reach only through the planner's overrides: rows C2C on one
+# LDS plane (ASDFFT_ROWS_SPLIT=1) and the 2048-point column pass with 8 columns per workgroup
+# or a 64-point register budget (ASDFFT_SPLIT_TPB / ASDFFT_SPLIT_EMAX)
+OPT_IN_SPLIT = [
+    ({"ASDFFT_ROWS_SPLIT": "1"}, (1024,), 37, "variant=20"),
+    ({"ASDFFT_ROWS_SPLIT": "1"}, (4096,), 9, "variant=20"),
+    ({"ASDFFT_SPLIT_TPB": "8"}, (2048, 40), 1, "variant=15 n=2048 count=40 radices=16x16x8 emax=32 T=64 tpb=8"),
+    ({"ASDFFT_SPLIT_EMAX": "64", "ASDFFT_SPLIT_TPB": "8"}, (2048, 24), 1,
+     "variant=15 n=2048 count=24 radices=16x16x8 emax=64 T=32 tpb=8"),
+    ({"ASDFFT_SPLIT_EMAX": "64"}, (2048, 20), 2, "variant=15 n=2048 count=40 radices=16x16x8 emax=64 T=32 tpb=16"),
+]
+
+
+@pytest.mark.parametrize("env,dims,batch,expect", OPT_IN_SPLIT, ids=str)
+def test_opt_in_one_plane_kernels(torch_dev, oracle, monkeypatch, env, dims, batch, expect):
+    import torch
+
+    from sip_amd.fft import Plan
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    rng = np.random.default_rng(sum(dims) + batch)
+    x = _uniform_c(rng, (batch,) + dims)
+    for inverse in (False, True):
+        p = Plan("c2c", dims, batch, inverse)
+        try:
+            assert expect in p.describe() and "static=" in p.describe(), p.describe()
+            got = p.execute(torch.from_numpy(x).to(torch_dev)).cpu().numpy()
+        finally:
+            p.close()
+        want = oracle.c2c_nd(x, len(dims), inverse)
+        err = per_row_rel(got, want, len(dims)).max()
+        assert err <= tol_for(dims), (env, dims, inverse, err)
 
 
 def test_vertical_2048_columns(torch_dev, oracle):
