@@ -17,6 +17,7 @@ constexpr int kPrimes[] = {2, 3, 5, 7, 11, 13, 17, 19};
 constexpr int kNumPrimes = 8;
 constexpr size_t kLdsLimit = 160 * 1024;      // gfx950 LDS per workgroup
 constexpr size_t kLdsPreferred = 64 * 1024;   // keep >= 2 workgroups per CU when batching rows
+constexpr int64_t kNumCUs = 256;               // MI355X compute units (8 XCDs x 32)
 
 struct RadixInfo {
     int radix;
@@ -138,6 +139,13 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
         const int maxb = max_block_for(emax);
         const int target = want_tpb > 0 ? want_tpb : std::max(1, 256 / T);
         int tpb = std::min(target, maxb / T);
+        if (want_tpb <= 0) {
+            // small batches: fewer transforms per workgroup until the launch has two workgroups
+            // per CU (1000 x 256: 4.77 us at one transform per workgroup, 5.44 us at two, whose
+            // 128 workgroups leave half the CUs idle), keeping at least one full wavefront
+            const int64_t spread = std::max<int64_t>((64 + T - 1) / T, count / (2 * kNumCUs));
+            tpb = (int)std::min<int64_t>(tpb, spread);
+        }
         tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, count));
         const bool need_lds = radices.size() > 1 || epi_lds;
         const int64_t tstride = pad16(n) + 1;
