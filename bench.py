@@ -571,18 +571,43 @@ VENDOR_CASES = [
 ]
 
 
-def bench_vendor(torch, dev, stream, reps):
-    """Vendor comparator: the same transforms through torch.fft, which runs hipFFT/rocFFT on
-    ROCm, beside this library's C-ABI exec.  Both are event-timed on one stream over the same
-    ring of inputs (ring_copies: each side spans at least RING_BYTES, so the device cache
-    cannot serve a repetition).  The
-    C-ABI calls write to the ring's outputs; torch.fft allocates each output, and the previous
-    output of the same ring slot is released just before, so its writes rotate over as many
-    blocks.  torch's C2R clones its input first (rocFFT's C2R overwrites its input): that copy
-    is part of the torch figure.  agree = relative L2 between the two outputs."""
+# hipFFT (include/hipfft/hipfft.h): transform types and the forward direction
+HIPFFT_C2C, HIPFFT_R2C, HIPFFT_C2R, HIPFFT_FORWARD = 0x29, 0x2A, 0x2C, -1
+
+
+def _hipfft(torch):
+    """The hipFFT library torch.fft itself calls (torch's bundled copy), through ctypes."""
+    path = os.path.join(os.path.dirname(torch.__file__), "lib", "libhipfft.so")
+    H = ctypes.CDLL(path)
+    vp, i = ctypes.c_void_p, ctypes.c_int
+    H.hipfftPlanMany.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(i), vp, i, i, vp, i, i, i, i]
+    H.hipfftSetStream.argtypes = [vp, vp]
+    H.hipfftExecC2C.argtypes = [vp, vp, vp, i]
+    H.hipfftExecR2C.argtypes = [vp, vp, vp]
+    H.hipfftExecC2R.argtypes = [vp, vp, vp]
+    H.hipfftDestroy.argtypes = [vp]
+    for f in ("hipfftPlanMany", "hipfftSetStream", "hipfftExecC2C", "hipfftExecR2C", "hipfftExecC2R",
+              "hipfftDestroy"):
+        getattr(H, f).restype = i
+    return H
+
+
+def bench_vendor(torch, dev, stream, reps, with_torch=True):
+    """Vendor comparator: the same transforms through rocFFT, called two ways, beside this
+    library's C-ABI exec, all event-timed on one stream over the same ring of (input, output)
+    pairs (ring_copies: each side spans at least RING_BYTES, so the device cache cannot serve a
+    repetition):
+      hipfft: hipFFT's plan-many / exec entry points called directly on the ring's buffers (the
+        library torch.fft uses, without torch: no extra device work);
+      torch: torch.fft, which copies its input before each rocFFT call and allocates each
+        output (the previous output of the same ring slot is released just before, so its writes
+        rotate over as many blocks).
+    rocFFT's C2R overwrites its input, so the hipfft path runs last on the ring.  agree_* =
+    relative L2 between this library's output and the other path's, on the same input."""
     from sip_amd import _native as N
     from sip_amd.fft import Plan
     L = N.lib()
+    H = _hipfft(torch)
     res = {}
     for name, kind, dims, batch, inv in VENDOR_CASES:
         rank = len(dims)
@@ -597,6 +622,7 @@ def bench_vendor(torch, dev, stream, reps):
                 x0 = torch.fft.rfft(torch.randn((batch, n), generator=g, device=dev)).contiguous()
             else:
                 x0 = make_inputs(torch, dev, kind, dims, batch, len(name))
+            pristine = x0.clone()
             out_dtype = torch.float32 if kind == "c2r" else torch.complex64
             y0 = torch.empty(plan.out_shape(), dtype=out_dtype, device=dev)
             copies = ring_copies(x0, y0)
@@ -614,6 +640,23 @@ def bench_vendor(torch, dev, stream, reps):
                 tf = torch.fft.fft
             else:
                 tf = lambda x: torch.fft.fftn(x, dim=tuple(range(-rank, 0)))
+            hp = ctypes.c_void_p()
+            nn = (ctypes.c_int * rank)(*dims)
+            htype = {"c2c": HIPFFT_C2C, "r2c": HIPFFT_R2C, "c2r": HIPFFT_C2R}[kind]
+            if H.hipfftPlanMany(ctypes.byref(hp), rank, nn, None, 1, 0, None, 1, 0, htype, batch) != 0:
+                raise RuntimeError(f"{name}: hipfftPlanMany failed")
+            if H.hipfftSetStream(hp, ctypes.c_void_p(stream.cuda_stream)) != 0:
+                raise RuntimeError(f"{name}: hipfftSetStream failed")
+
+            def hip_exec(x, y):
+                if kind == "c2c":
+                    st = H.hipfftExecC2C(hp, x, y, HIPFFT_FORWARD)
+                elif kind == "r2c":
+                    st = H.hipfftExecR2C(hp, x, y)
+                else:
+                    st = H.hipfftExecC2R(hp, x, y)
+                if st != 0:
+                    raise RuntimeError(f"{name}: hipfftExec returned {st}")
             torch.cuda.synchronize()
             cur = [0]
 
@@ -623,29 +666,49 @@ def bench_vendor(torch, dev, stream, reps):
                 if st != 0:
                     raise RuntimeError(f"{name}: exec returned {st}")
             ms_ours = event_time_ms(torch, stream, ours, reps)
-            outs = [None] * copies
 
-            def vendor():
+            def rel(a, b):
+                return float(f"{(torch.linalg.vector_norm((a - b).flatten()) / torch.linalg.vector_norm(b.flatten())).item():.3g}")
+            entry = {"ours_us": round(ms_ours * 1e3, 2),
+                     "ours_hbm_frac": round(alg_bytes(kind, dims, batch) / ms_ours / 1e6 / HBM_PEAK_GBS, 4)}
+            if with_torch:
+                outs = [None] * copies
+
+                def via_torch():
+                    i = cur[0] = (cur[0] + 1) % copies
+                    outs[i] = None
+                    outs[i] = tf(xs[i])
+                ms_torch = event_time_ms(torch, stream, via_torch, reps)
+                ours()
+                i = cur[0]
+                entry.update({"torch_fft_us": round(ms_torch * 1e3, 2), "speedup_vs_torch_fft": round(ms_torch / ms_ours, 3),
+                              "agree_torch_fft": rel(ys[i], tf(xs[i]))})
+                del outs
+
+            def via_hipfft():
                 i = cur[0] = (cur[0] + 1) % copies
-                outs[i] = None
-                outs[i] = tf(xs[i])
-            ms_vendor = event_time_ms(torch, stream, vendor, reps)
-            # agreement on one ring slot: both outputs of the same input
-            ours()
-            i = cur[0]
-            ref = tf(xs[i])
+                hip_exec(xs[i].data_ptr(), ys[i].data_ptr())
+            ms_hip = event_time_ms(torch, stream, via_hipfft, reps)
+            # agreement on fresh copies of the input (rocFFT's C2R overwrites its input)
+            xs[0].copy_(pristine)
+            xs[1 % copies].copy_(pristine)
+            y_ours = torch.empty_like(y0)
+            to = acl(y_ours)
+            st = fn(plan.handle, tins[0].ptr, to.ptr)
+            to.close()
+            if st != 0:
+                raise RuntimeError(f"{name}: exec returned {st}")
+            hip_exec(xs[1 % copies].data_ptr(), ys[1 % copies].data_ptr())
             torch.cuda.synchronize()
-            agree = (torch.linalg.vector_norm((ys[i] - ref).flatten()) /
-                     torch.linalg.vector_norm(ref.flatten())).item()
+            entry.update({"rocfft_us": round(ms_hip * 1e3, 2), "speedup_vs_rocfft": round(ms_hip / ms_ours, 3),
+                          "rocfft_hbm_frac": round(alg_bytes(kind, dims, batch) / ms_hip / 1e6 / HBM_PEAK_GBS, 4),
+                          "agree_rocfft": rel(y_ours, ys[1 % copies]), "ring_pairs": copies})
+            H.hipfftDestroy(hp)
             for t in tins + touts:
                 t.close()
             plan.close()
-            del xs, ys, outs, ref
-        res[name] = {"ours_us": round(ms_ours * 1e3, 2), "rocfft_us": round(ms_vendor * 1e3, 2),
-                     "speedup": round(ms_vendor / ms_ours, 3),
-                     "ours_hbm_frac": round(alg_bytes(kind, dims, batch) / ms_ours / 1e6 / HBM_PEAK_GBS, 4),
-                     "rocfft_hbm_frac": round(alg_bytes(kind, dims, batch) / ms_vendor / 1e6 / HBM_PEAK_GBS, 4),
-                     "agree_rel_l2": float(f"{agree:.3g}"), "ring_pairs": copies}
+            del xs, ys, y_ours, pristine
+        res[name] = entry
     torch.cuda.empty_cache()
     return res
 
@@ -771,6 +834,8 @@ def main():
     ap.add_argument("--istft-only", action="store_true", help="run only the ISTFT form")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocFFT (torch.fft) comparator form")
     ap.add_argument("--vendor-only", action="store_true", help="run only the rocFFT (torch.fft) comparator form")
+    ap.add_argument("--vendor-no-torch", action="store_true",
+                    help="comparator form without the torch.fft path (direct hipFFT only; for kernel traces)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
@@ -802,7 +867,8 @@ def main():
         print(json.dumps({"forms": {"istft": bench_istft(torch, dev, stream, max(5, a.kernel_reps))}}))
         return
     if a.vendor_only:
-        print(json.dumps({"forms": {"vendor": bench_vendor(torch, dev, stream, max(5, a.kernel_reps))}}))
+        print(json.dumps({"forms": {"vendor": bench_vendor(torch, dev, stream, max(5, a.kernel_reps),
+                                                           not a.vendor_no_torch)}}))
         return
     L = N.lib()
     entries = [w for w in WORKLOAD if a.only is None or w[0] == a.only or
