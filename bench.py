@@ -578,6 +578,8 @@ HIPFFT_C2C, HIPFFT_R2C, HIPFFT_C2R, HIPFFT_FORWARD = 0x29, 0x2A, 0x2C, -1
 def _hipfft(torch):
     """The hipFFT library torch.fft itself calls (torch's bundled copy), through ctypes."""
     path = os.path.join(os.path.dirname(torch.__file__), "lib", "libhipfft.so")
+    if not os.path.exists(path):
+        return None
     H = ctypes.CDLL(path)
     vp, i = ctypes.c_void_p, ctypes.c_int
     H.hipfftPlanMany.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(i), vp, i, i, vp, i, i, i, i]
@@ -608,7 +610,7 @@ def bench_vendor(torch, dev, stream, reps, with_torch=True):
     from sip_amd.fft import Plan
     L = N.lib()
     H = _hipfft(torch)
-    res = {}
+    res = {} if H is not None else {"hipfft": "unavailable (torch ships no libhipfft.so)"}
     for name, kind, dims, batch, inv in VENDOR_CASES:
         rank = len(dims)
         with torch.cuda.stream(stream):
@@ -641,12 +643,13 @@ def bench_vendor(torch, dev, stream, reps, with_torch=True):
             else:
                 tf = lambda x: torch.fft.fftn(x, dim=tuple(range(-rank, 0)))
             hp = ctypes.c_void_p()
-            nn = (ctypes.c_int * rank)(*dims)
-            htype = {"c2c": HIPFFT_C2C, "r2c": HIPFFT_R2C, "c2r": HIPFFT_C2R}[kind]
-            if H.hipfftPlanMany(ctypes.byref(hp), rank, nn, None, 1, 0, None, 1, 0, htype, batch) != 0:
-                raise RuntimeError(f"{name}: hipfftPlanMany failed")
-            if H.hipfftSetStream(hp, ctypes.c_void_p(stream.cuda_stream)) != 0:
-                raise RuntimeError(f"{name}: hipfftSetStream failed")
+            if H is not None:
+                nn = (ctypes.c_int * rank)(*dims)
+                htype = {"c2c": HIPFFT_C2C, "r2c": HIPFFT_R2C, "c2r": HIPFFT_C2R}[kind]
+                if H.hipfftPlanMany(ctypes.byref(hp), rank, nn, None, 1, 0, None, 1, 0, htype, batch) != 0:
+                    raise RuntimeError(f"{name}: hipfftPlanMany failed")
+                if H.hipfftSetStream(hp, ctypes.c_void_p(stream.cuda_stream)) != 0:
+                    raise RuntimeError(f"{name}: hipfftSetStream failed")
 
             def hip_exec(x, y):
                 if kind == "c2c":
@@ -684,6 +687,13 @@ def bench_vendor(torch, dev, stream, reps, with_torch=True):
                 entry.update({"torch_fft_us": round(ms_torch * 1e3, 2), "speedup_vs_torch_fft": round(ms_torch / ms_ours, 3),
                               "agree_torch_fft": rel(ys[i], tf(xs[i]))})
                 del outs
+
+            if H is None:
+                for t in tins + touts:
+                    t.close()
+                plan.close()
+                res[name] = entry
+                continue
 
             def via_hipfft():
                 i = cur[0] = (cur[0] + 1) % copies
