@@ -455,13 +455,20 @@ void Server::scheduler_loop()
             return false;
         });
         if (stop) break;
-        // admission: arrival order while the budget and the slot's tables allow
+        // admission: arrival order while the budget and the slot's tables allow.  The pending
+        // requests are shared among the idle slots (each takes its share now, the others take
+        // theirs as the scheduler comes round), so under light load every request gets a slot
+        // and a stream of its own and the batches grow only when requests queue up
         std::vector<Req*> batch;
         std::vector<uint64_t> tickets;
         size_t add = 0;
         int64_t transforms = 0;
         const size_t budget = cfg.hbm_budget - std::min(cfg.hbm_budget, resident);
-        while (!pending.empty() && (int)batch.size() < cfg.max_batch) {
+        size_t idle_slots = 0;
+        for (const auto& sl : slots) idle_slots += sl->busy ? 0 : 1;
+        const size_t share = (pending.size() + std::max<size_t>(idle_slots, 1) - 1) / std::max<size_t>(idle_slots, 1);
+        const size_t limit = std::min<size_t>((size_t)cfg.max_batch, std::max<size_t>(share, 1));
+        while (!pending.empty() && batch.size() < limit) {
             Req& q = reqs[pending.front()];
             // with nothing in flight the head request is admitted even if resident state
             // (grown workspaces) has eaten into the budget since it was submitted
