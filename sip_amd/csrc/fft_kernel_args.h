@@ -46,6 +46,7 @@ enum Epilogue : int {
     kEpiTransposePlanar = 7,  // kEpiTranspose into two float planes
     kEpiRowsOut = 8,   // column mapping: each transform stored contiguously through LDS
     kEpiShift = 9,     // complex store through the output shift (roll_out, out_col_roll), times out_scale
+    kEpiPairRadix2 = 10,  // rows n and n + o_split of a pair, through LDS: u + v and (u - v) W^n (tw_aux)
 };
 
 struct PassDesc {
@@ -86,6 +87,9 @@ struct FftArgs {
     int roll_in, roll_out;
     int64_t in_o_mod, in_o_roll, out_col_roll;
     float out_scale;
+    // split outer index (o_split > 0): the outer index o = t / cols addresses
+    // (o % o_split) * outer + (o / o_split) * outer2 instead of o * outer
+    int64_t o_split, in_outer2, out_outer2;
     PassDesc pass[kMaxPasses];
 };
 

@@ -317,18 +317,27 @@ struct StepIo {
     int64_t cols = 1;
     int64_t in_inner = 0, in_outer = 0, in_stride = 1;
     int64_t out_inner = 0, out_outer = 0, out_stride = 1;
+    int64_t o_split = 0, in_outer2 = 0, out_outer2 = 0;  // split outer index (FftArgs::o_split)
 };
 
 bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, int conj_in, int conj_out,
                    int want_tpb, Step* st, std::string* why)
 {
     Config c;
-    const bool epi_lds = (variant == kVarR2CEven || variant == kVarRowsTranspose || variant == kVarRowsTransposeSep);
+    const bool pair = (variant == kVarRowsPair2);
+    const bool epi_lds = (variant == kVarR2CEven || variant == kVarRowsTranspose || variant == kVarRowsTransposeSep ||
+                          pair);
+    // the pair variant has the row C2C kernel's geometry, two transforms per workgroup
     const bool ok = cols_mapping ? cols_config(n, io.count, variant, &c)
-                                 : rows_config(n, io.count, epi_lds, want_tpb, variant, &c);
-    if (!ok) {
+                                 : rows_config(n, io.count, epi_lds, pair ? 2 : want_tpb,
+                                               pair ? (int)kVarRowsC2C : variant, &c);
+    if (!ok || (pair && (c.tpb != 2 || c.radices.size() < 2))) {
         *why = "no single-workgroup schedule for length " + std::to_string(n);
         return false;
+    }
+    if (pair) {
+        c.split = false;
+        c.lds = (size_t)(8 * c.tpb * c.tstride);
     }
     std::vector<int> offsets;
     std::vector<float> tab = stockham_table((int)n, c.radices, &offsets);
@@ -356,6 +365,9 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     a.xcd_remap = xcd_remap_for(cols_mapping, c.tpb);
     a.in_o_mod = 1;
     a.out_scale = 1.0f;
+    a.o_split = io.o_split;
+    a.in_outer2 = io.in_outer2;
+    a.out_outer2 = io.out_outer2;
     fill_passes(c, &a, &offsets);
     st->direct = false;
     st->shape.variant = !c.split ? variant : split_variant_of(variant);
@@ -764,6 +776,75 @@ void mfma_rows_1024(int64_t count, bool inverse, BuiltPlan* bp)
 
 int max_block_for(int emax) { return emax <= 16 ? 256 : 512; }
 
+namespace {
+
+// 2D C2C with the first radix-2 stage of the columns (length X) folded into the row pass
+// (ASDFFT_PAIR2D=1): rows n and n + X/2 of a slice share a workgroup, which stores u + v to row
+// n and (u - v) W_X^n to row n + X/2 of the workspace; one column launch then transforms the
+// two halves as length-X/2 columns, storing point k of half h to row 2k + h of the output.
+bool pair2d_wanted(int64_t X, int64_t Y)
+{
+    const char* e = getenv("ASDFFT_PAIR2D");
+    return e != nullptr && e[0] == '1' && X % 2 == 0 && X / 2 >= 2 && Y >= 2;
+}
+
+bool pair2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
+{
+    const int64_t h = X / 2;
+    std::string why;
+    BuiltPlan bp;
+    // W_X^n, n < X/2, forward sign, from long double
+    std::vector<float> tab;
+    static const long double kTwoPi = 6.283185307179586476925286766559005768L;
+    for (int64_t m = 0; m < h; m++) {
+        const long double ang = kTwoPi * (long double)m / (long double)X;
+        tab.push_back((float)cosl(ang));
+        tab.push_back((float)(-sinl(ang)));
+    }
+    TablePtr pt = upload_table("pair2d:" + std::to_string(X), tab);
+    Step rows;
+    rows.src = kBufIn;
+    rows.dst = kBufWork;
+    rows.bytes = 16.0 * (double)(X * Y * B);
+    StepIo ri;
+    ri.count = B * X;
+    ri.cols = 2;
+    ri.in_inner = h * Y;
+    ri.in_outer = Y;
+    ri.in_outer2 = X * Y;
+    ri.out_inner = h * Y;
+    ri.out_outer = Y;
+    ri.out_outer2 = X * Y;
+    ri.o_split = h;
+    if (!stockham_step(Y, kVarRowsPair2, false, ri, inv, 0, 2, &rows, &why) || rows.shape.static_id < 0) return false;
+    rows.args.tw_aux = pt ? pt->dev : nullptr;
+    rows.tables_ready = rows.tables_ready && pt != nullptr;
+    if (pt) rows.tables.push_back(pt);
+    Step cols;
+    cols.src = kBufWork;
+    cols.dst = kBufOut;
+    cols.bytes = 16.0 * (double)(X * Y * B);
+    StepIo ci;
+    ci.count = B * 2 * Y;
+    ci.cols = Y;
+    ci.in_inner = 1;
+    ci.in_outer = h * Y;
+    ci.in_outer2 = X * Y;
+    ci.in_stride = Y;
+    ci.out_inner = 1;
+    ci.out_outer = Y;
+    ci.out_outer2 = X * Y;
+    ci.out_stride = 2 * Y;
+    ci.o_split = 2;
+    if (!stockham_step(h, kVarColsC2C, true, ci, 0, inv, 0, &cols, &why)) return false;
+    out->steps.push_back(std::move(rows));
+    out->steps.push_back(std::move(cols));
+    out->workspace = std::max(out->workspace, (size_t)(X * Y * B * 8));
+    return true;
+}
+
+}  // namespace
+
 std::vector<int> choose_radices(int64_t n)
 {
     if (n <= 1) return {};
@@ -1029,6 +1110,7 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
 
     if (spec.kind == Kind::C2C) {
         if (L > kMaxSingleKernel) return "last-axis length above " + std::to_string(kMaxSingleKernel);
+        if (nd == 2 && pair2d_wanted(spec.dims[0], L) && pair2d(spec.dims[0], L, B, inv, out)) return "";
         if (!c2c_rows(L, rows, inv, kBufIn, kBufOut, out, &why)) return why;
         for (size_t d = nd - 1; d-- > 0;) {
             if (!col_pass(d, kBufOut, kBufOut)) return why;

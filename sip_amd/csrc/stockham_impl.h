@@ -396,8 +396,8 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
     constexpr int B = EMAX / R;
     static_assert(B >= 1, "radix larger than the per-thread register budget");
-    constexpr bool kEpiViaLds =
-        (EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar || EPI == kEpiRowsOut);
+    constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar ||
+                                 EPI == kEpiRowsOut || EPI == kEpiPairRadix2);
     if constexpr (RT) {
         asm volatile("" : "+s"(g.n), "+s"(g.T));
         asm volatile("" : "+v"(tid));
@@ -579,8 +579,12 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
             co += a.out_col_roll;
             if (co >= a.cols) co -= a.cols;
         }
-        const int64_t ib = io.col * a.in_inner + oi * a.in_outer;
-        const int64_t ob = co * a.out_inner + o * a.out_outer;
+        int64_t ib = io.col * a.in_inner + oi * a.in_outer;
+        int64_t ob = co * a.out_inner + o * a.out_outer;
+        if (a.o_split > 0) {  // uniform
+            ib = io.col * a.in_inner + (oi % a.o_split) * a.in_outer + (oi / a.o_split) * a.in_outer2;
+            ob = co * a.out_inner + (o % a.o_split) * a.out_outer + (o / a.o_split) * a.out_outer2;
+        }
         if constexpr (PRO == kProReal) {
             io.in = reinterpret_cast<const float*>(a.in) + ib;
         } else if constexpr (PRO == kProPlanar) {
@@ -656,6 +660,25 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                     obr[i * s] = lre[at];
                     obi[i * s] = sgn * lim[at];
                 }
+            }
+        }
+    } else if constexpr (EPI == kEpiPairRadix2) {
+        // rows mapping, two transforms per workgroup: 2 blk (row n) and 2 blk + 1 (row n + X/2
+        // of the same slice), n = blk % o_split.  First radix-2 stage of the length-X column
+        // transforms (decimation in frequency): row n <- u + v, row n + X/2 <- (u - v) W_X^n,
+        // W_X^n = tw_aux[n] (forward sign; the inverse comes from the conjugations)
+        const int64_t t0 = blk * 2;
+        if (t0 + 1 < a.count) {
+            const float2 w = ldg2(a.tw_aux, blk % a.o_split);
+            const int64_t ob = (blk % a.o_split) * a.out_outer + (blk / a.o_split) * a.out_outer2;
+            float2* out0 = reinterpret_cast<float2*>(a.out) + ob;
+            float2* out1 = out0 + a.out_inner;
+            for (int i = threadIdx.x; i < g.n; i += g.T * 2) {
+                const int at0 = lds_at<MAP>(g, 0, i), at1 = lds_at<MAP>(g, 1, i);
+                const float2 u = make_float2(lre[at0], lim[at0]);
+                const float2 v = make_float2(lre[at1], lim[at1]);
+                out0[i] = conj_if(cadd(u, v), io.cout);
+                out1[i] = conj_if(cmul(csub(u, v), w), io.cout);
             }
         }
     } else if constexpr (EPI == kEpiRowsOut) {
@@ -929,6 +952,7 @@ template <> struct VariantTraits<kVarColsToRows> { static constexpr int map = kM
 template <> struct VariantTraits<kVarRowsShiftIn> { static constexpr int map = kMapRows, pro = kProShift, epi = kEpiComplex; };
 template <> struct VariantTraits<kVarRowsShiftBoth> { static constexpr int map = kMapRows, pro = kProShift, epi = kEpiShift; };
 template <> struct VariantTraits<kVarColsShiftOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiShift; };
+template <> struct VariantTraits<kVarRowsPair2> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiPairRadix2; };
 
 template <typename K>
 hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipStream_t stream)

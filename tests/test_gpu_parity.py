@@ -343,6 +343,34 @@ def test_opt_in_one_plane_kernels(torch_dev, oracle, monkeypatch, env, dims, bat
         assert err <= tol_for(dims), (env, dims, inverse, err)
 
 
+# 2D C2C with the columns' first radix-2 stage folded into the row pass (ASDFFT_PAIR2D=1):
+# rows n and n + X/2 in one workgroup, then length-X/2 columns stored to rows 2k + h
+PAIR2D = [((2048, 2048), 1), ((2048, 2048), 2), ((2048, 1024), 2), ((1024, 1024), 3), ((1024, 2048), 1)]
+
+
+@pytest.mark.parametrize("dims,batch", PAIR2D, ids=str)
+@pytest.mark.parametrize("inverse", [False, True])
+def test_pair2d_plans(torch_dev, oracle, monkeypatch, dims, batch, inverse):
+    import torch
+
+    from sip_amd.fft import Plan
+    monkeypatch.setenv("ASDFFT_PAIR2D", "1")
+    rng = np.random.default_rng(sum(dims) + batch + inverse)
+    x = _uniform_c(rng, (batch,) + dims)
+    p = Plan("c2c", dims, batch, inverse)
+    try:
+        assert "variant=21" in p.describe(), p.describe()
+        xt = torch.from_numpy(x).to(torch_dev)
+        got = p.execute(xt).cpu().numpy()
+        assert np.array_equal(xt.cpu().numpy(), x)  # the input is left as it was
+    finally:
+        p.close()
+    last = batch - 1
+    want = oracle.c2c_nd(x[last:], 2, inverse)
+    err = per_row_rel(got[last:], want, 2).max()
+    assert err <= tol_for(dims), (dims, batch, inverse, err)
+
+
 def test_vertical_2048_columns(torch_dev, oracle):
     import torch
 
