@@ -292,10 +292,12 @@ def test_specialised_kernels_match_runtime_kernel(torch_dev, oracle, monkeypatch
     assert checked >= 20
 
 
-# long column passes (the one-plane LDS exchange kernel: 2048 and 1000 points, 16 columns
-# per workgroup): full and partial workgroups, 2D, 3D and VERTICAL 1D
+# long column passes (the one-plane LDS exchange kernel: 2048, 1024 and 1000 points with 16
+# columns per workgroup, 4096 points with 8): full and partial workgroups, 2D, 3D and VERTICAL 1D
 LONG_COLUMNS = [("c2c", (2048, 64), 1, False), ("c2c", (2048, 40), 2, True), ("c2c", (2048, 4, 8), 1, False),
-                ("c2c", (2048, 24), 1, True), ("c2c", (1000, 40), 2, False), ("c2c", (1000, 1000), 1, True)]
+                ("c2c", (2048, 24), 1, True), ("c2c", (1000, 40), 2, False), ("c2c", (1000, 1000), 1, True),
+                ("c2c", (1024, 40), 1, False), ("c2c", (1024, 1024), 1, True), ("c2c", (4096, 40), 1, False),
+                ("c2c", (4096, 12), 1, True)]
 
 
 @pytest.mark.parametrize("kind,dims,batch,inverse", LONG_COLUMNS, ids=str)
