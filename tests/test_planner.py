@@ -19,10 +19,21 @@ def field(line, name):
 @pytest.mark.parametrize("n,radices", [(4096, "16x16x16"), (1000, "10x10x10"), (2187, "9x9x9x3"),
                                        (256, "16x16"), (16, "16"), (17, "17"), (6, "6"), (2048, "16x16x8")])
 def test_radix_schedule(n, radices):
-    p = Plan("c2c", (n,), 4)
+    p = Plan("c2c", (n,), 2048)  # large enough that the small-launch order rule does not apply
     s = steps(p.describe())
     assert len(s) == 1 and field(s[0], "radices") == radices
     p.close()
+
+
+def test_small_launches_take_the_smallest_radix_first():
+    """Row C2C launches of at most 16 MiB of input whose radices are all below 12 run the smallest
+    radix first (2187 x 256: 3x9x9x9); larger launches and sequences with radix 12 or 16 keep the
+    fewest-pass order."""
+    for n, b, want in [(2187, 256, "3x9x9x9"), (2187, 2048, "9x9x9x3"), (6000, 256, "6x10x10x10"),
+                       (1000, 256, "10x10x10"), (4096, 256, "16x16x16"), (1536, 256, "16x16x6")]:
+        p = Plan("c2c", (n,), b)
+        assert field(steps(p.describe())[0], "radices") == want, (n, b)
+        p.close()
 
 
 def test_one_launch_per_single_kernel_length():
