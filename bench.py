@@ -11,11 +11,17 @@ Inputs are synthetic uniform[-1, 1] (the R2C input is Gaussian noise plus three
 random tones per row), generated on the device from fixed seeds.
 
 value = aggregate GFLOP/s over the step, FLOP = 5 N log2 N per transform
-(N = total points; the same formula for R2C/C2R).  Timing uses HIP events on
-the stream the plans launch on, bracketed by device synchronisation.  The JSON
-line also carries the roofline of the dominant kernel launch (algorithmic
-bytes / its average event-timed duration vs 8 TB/s HBM) and the CPU FP64
-oracle's throughput on a bounded sample of the same workload.
+(N = total points; the same formula for R2C/C2R, whose conventional 2.5 N log2 N
+figure is reported beside it).  The step is timed with HIP events on the stream
+the plans launch on, bracketed by device synchronisation.  Each configuration
+and each kernel launch is also timed per sample, as the reference's benchmark
+does (start event, exec, end event; --samples of them, median), with the stream
+held by a spin kernel while each sample is enqueued so that the interval is
+device time.  The JSON line carries the roofline of the dominant kernel launch
+(algorithmic bytes / its mean per-sample duration vs 8 TB/s HBM), each
+configuration's relative L2 error against a float64 FFT on sampled transforms,
+a device-copy ceiling of the same bytes, and the reference's CPU golden (numpy
+FFT as its golden generators call it) timed on the host.
 """
 from __future__ import annotations
 
@@ -154,7 +160,114 @@ def event_time_ms(torch, stream, fn, reps):
     return start.elapsed_time(end) / reps
 
 
-def cpu_baseline(seconds_budget=20.0):
+_SLEEP_CYCLES_PER_US = []
+
+
+def _spin_cycles(torch, stream, us):
+    """torch.cuda._sleep cycles that hold `stream` for about `us` microseconds (calibrated once)."""
+    if not _SLEEP_CYCLES_PER_US:
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            torch.cuda._sleep(1000)
+            torch.cuda.synchronize()
+            s.record(stream)
+            torch.cuda._sleep(1 << 20)
+            e.record(stream)
+        torch.cuda.synchronize()
+        _SLEEP_CYCLES_PER_US.append((1 << 20) / max(s.elapsed_time(e) * 1e3, 1e-3))
+    return max(1, int(us * _SLEEP_CYCLES_PER_US[0]))
+
+
+def device_times_us(torch, stream, fn, samples=300, hold_us=80.0):
+    """Per-sample device time of fn() as the reference's benchmark takes it
+    (tests/standalone/fft1d_optimized/bench.cpp:176-190: start event, exec, end event, one
+    sample each; the median is reported).  Before each sample a spin kernel holds the stream
+    for hold_us, longer than the host needs to enqueue the two events and the exec, so the
+    events bracket queued work: the interval is device time, not host submission latency.
+    Returns {median, mean, p10, p90, n} in microseconds."""
+    cyc = _spin_cycles(torch, stream, hold_us)
+    with torch.cuda.stream(stream):
+        fn()  # untimed: first launches load their code objects
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(samples)]
+        for i, (s, e) in enumerate(ev):
+            torch.cuda._sleep(cyc)
+            s.record(stream)
+            fn()
+            e.record(stream)
+            if i % 64 == 63:
+                torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    t = sorted(s.elapsed_time(e) * 1e3 for s, e in ev)
+    n = len(t)
+    return {"median": t[n // 2] if n % 2 else 0.5 * (t[n // 2 - 1] + t[n // 2]), "mean": sum(t) / n,
+            "p10": t[n // 10], "p90": t[(9 * n) // 10], "n": n}
+
+
+def median_ms(torch, stream, fn, samples=300):
+    """Median per-sample device time of fn() in milliseconds (device_times_us)."""
+    return device_times_us(torch, stream, fn, samples)["median"] * 1e-3
+
+
+def copy_ceiling_us(torch, stream, x, y, samples=300):
+    """Median device time of a plain device-to-device copy moving the bytes of one exec of a
+    transform from x to y (torch's copy kernel, (|x| + |y|) / 2 bytes read and as many written,
+    so read + write equal the transform's algorithmic bytes), rotated over a ring larger than
+    the device cache like the transform itself."""
+    half = (x.numel() * x.element_size() + y.numel() * y.element_size()) // 2
+    half -= half % 16
+    copies = max(1, min(RING_MAX, -(-RING_BYTES // half)))
+    srcs = [torch.empty(half, dtype=torch.uint8, device=x.device) for _ in range(copies)]
+    dsts = [torch.empty(half, dtype=torch.uint8, device=x.device) for _ in range(copies)]
+    for t in srcs:
+        t.fill_(1)
+    cur = [0]
+
+    def run():
+        i = cur[0] = (cur[0] + 1) % copies
+        dsts[i].copy_(srcs[i])
+    us = device_times_us(torch, stream, run, samples)["median"]
+    del srcs, dsts
+    return us
+
+
+def sampled_error(torch, kind, dims, batch, inverse, x, y, rows=16):
+    """Relative L2 error per transform of y = exec(x) against numpy's float64 FFT (the
+    reference's golden arithmetic in double precision) on sampled transforms, outside any timed
+    region: up to `rows` evenly spaced transforms for 1D (first and last included), the first and
+    last slice for 2D, the whole volume for 3D.  Returns max and RMS over the sample and the
+    bound 1e-6 log2 N (N = points per transform)."""
+    import numpy as np
+    n = math.prod(dims)
+    if len(dims) == 1:
+        idx = sorted(set(int(round(v)) for v in np.linspace(0, batch - 1, min(rows, batch))))
+    elif len(dims) == 2:
+        idx = sorted({0, batch - 1})
+    else:
+        idx = list(range(batch))
+    xs = x[idx].cpu().numpy()
+    ys = y[idx].cpu().numpy()
+    axes = tuple(range(1, len(dims) + 1))
+    if kind == "c2c":
+        xd = xs.astype(np.complex128)
+        ref = np.fft.ifftn(xd, axes=axes, norm="forward") if inverse else np.fft.fftn(xd, axes=axes)
+    elif kind == "r2c":
+        ref = np.fft.rfftn(xs.astype(np.float64), axes=axes)
+        if inverse:
+            ref = np.conj(ref)
+    else:
+        ref = np.fft.irfftn(xs.astype(np.complex128), s=dims, axes=axes, norm="forward")
+        if not inverse:
+            ref = np.fft.hfft(xs.astype(np.complex128), n=dims[-1])
+    err = np.linalg.norm((ys.astype(ref.dtype) - ref).reshape(len(idx), -1), axis=1) / \
+        np.linalg.norm(ref.reshape(len(idx), -1), axis=1)
+    bound = 1e-6 * math.log2(n)
+    return {"max": float(f"{err.max():.3g}"), "rms": float(f"{math.sqrt(float(np.mean(err ** 2))):.3g}"),
+            "bound": float(f"{bound:.3g}"), "within_bound": bool(err.max() <= bound), "transforms": len(idx),
+            "vs": "numpy float64 FFT"}
+
+
+def cpu_oracle(seconds_budget=20.0):
     """FP64 oracle (single-threaded C) on a bounded sample of the same workload."""
     import numpy as np
 
@@ -187,8 +300,9 @@ def cpu_baseline(seconds_budget=20.0):
                       f"{total_flop / 1e9:.2f} GFLOP in {dt:.1f} s"}
 
 
-def cpu_numpy(max_threads=16):
-    """numpy.fft on the host cores over one full step of the coverage workload, two ways:
+def cpu_numpy(max_threads=16, runs=5):
+    """numpy.fft on the host cores over one full step of the coverage workload, two ways, each
+    timed as the median of `runs` runs after one untimed warm-up:
     golden: as the reference's unit-test golden generators call it, on one thread: one
       np.fft.fft / rfft / irfft(norm="forward") per row, cast to complex64
       (tests/ut/unittest/fft/1d/c2c/c2c_data/gen_data.py:42-65, r2c_data/gen_data.py:43,
@@ -250,14 +364,19 @@ def cpu_numpy(max_threads=16):
                 list(ex.map(work, zip(bounds[:-1], bounds[1:])))
 
     total_flop = sum(flops(dims, batch) for _, dims, batch, _ in data)
-    t0 = time.perf_counter()
-    golden()
-    t_golden = time.perf_counter() - t0
+
+    def median_s(f):
+        f()  # untimed warm-up
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2], ts
+
+    t_golden, ts_golden = median_s(golden)
     with cf.ThreadPoolExecutor(threads) as ex:
-        pooled(ex)  # untimed: thread start-up
-        t0 = time.perf_counter()
-        pooled(ex)
-        t_pool = time.perf_counter() - t0
+        t_pool, ts_pool = median_s(lambda: pooled(ex))
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -266,10 +385,18 @@ def cpu_numpy(max_threads=16):
                 break
     except OSError:
         pass
-    return {"golden": {"gflops": round(total_flop / t_golden / 1e9, 3), "s_per_step": round(t_golden, 3),
-                       "threads": 1},
+    golden_gflops = total_flop / t_golden / 1e9
+    return {"baseline": {"value": round(golden_gflops, 3), "unit": "GFLOP/s", "cores": 1, "kind": "reference",
+                         "sample": "the reference's CPU golden: numpy.fft called as its golden generators call it "
+                                   "(one np.fft.fft / rfft / irfft(norm='forward') per row cast to complex64, "
+                                   "gen_data.py:42-65; fft2 / fftn for 2D / 3D) over one full step of the same "
+                                   f"workload (same shapes and batches, {total_flop / 1e9:.2f} GFLOP, 5 N log2 N), "
+                                   f"one thread; median of {runs} runs after one warm-up; numpy {np.__version__}; "
+                                   f"{model}",
+                         "s_per_step": round(t_golden, 3), "runs_s": [round(x, 3) for x in ts_golden],
+                         "cpu_model": model, "numpy": np.__version__},
             "threads": {"gflops": round(total_flop / t_pool / 1e9, 3), "s_per_step": round(t_pool, 3),
-                        "threads": threads},
+                        "threads": threads, "runs_s": [round(x, 3) for x in ts_pool]},
             "cpu_model": model, "cpus_visible": os.cpu_count(), "numpy": np.__version__,
             "sample": "one full step of the coverage workload (same shapes and batches), seeded numpy inputs"}
 
@@ -348,7 +475,7 @@ def bench_ragged(torch, dev, stream, reps, warmup):
             for _ in range(warmup):
                 fn()
             torch.cuda.synchronize()
-            times[name] = event_time_ms(torch, stream, fn, reps)
+            times[name] = median_ms(torch, stream, fn, reps)
         # ragged output equals the per-size plans' output
         run_plain()
         torch.cuda.synchronize()
@@ -486,7 +613,7 @@ def bench_precision(torch, dev, stream, reps):
                 y = torch.empty_like(x)
                 p.bind_workspace(dev)
                 p.bind_stream(stream.cuda_stream)
-                ms = event_time_ms(torch, stream, lambda: p.execute(x, y), reps)
+                ms = median_ms(torch, stream, lambda: p.execute(x, y), reps)
                 got = y[torch.as_tensor(rows, device=dev)].cpu().numpy().astype(np.complex128)
                 err = np.linalg.norm(got - gold, axis=1) / np.linalg.norm(gold, axis=1)
                 entry[prec] = {"us": round(ms * 1e3, 2), "gflops": round(flops((n,), batch) / ms / 1e6, 1),
@@ -525,7 +652,7 @@ def bench_separated(torch, dev, stream, reps):
                 st = L.asdFftExecC2CSeparated(ps.handle, *[t.ptr for t in ts])
                 if st != 0:
                     raise RuntimeError(f"asdFftExecC2CSeparated returned {st}")
-            ms = event_time_ms(torch, stream, run_sep, reps)
+            ms = median_ms(torch, stream, run_sep, reps)
             for t in ts:
                 t.close()
             entry["separated"] = {"us": round(ms * 1e3, 2), "gflops": round(flops(dims, batch) / ms / 1e6, 1),
@@ -543,7 +670,7 @@ def bench_separated(torch, dev, stream, reps):
                 st = L.asdFftExecC2C(pc.handle, ts[0].ptr, ts[1].ptr)
                 if st != 0:
                     raise RuntimeError(f"asdFftExecC2C returned {st}")
-            ms = event_time_ms(torch, stream, run_c2c, reps)
+            ms = median_ms(torch, stream, run_c2c, reps)
             for t in ts:
                 t.close()
             entry["interleaved"] = {"us": round(ms * 1e3, 2), "gflops": round(flops(dims, batch) / ms / 1e6, 1),
@@ -668,7 +795,7 @@ def bench_vendor(torch, dev, stream, reps, with_torch=True):
                 st = fn(plan.handle, tins[i].ptr, touts[i].ptr)
                 if st != 0:
                     raise RuntimeError(f"{name}: exec returned {st}")
-            ms_ours = event_time_ms(torch, stream, ours, reps)
+            ms_ours = median_ms(torch, stream, ours, reps)
 
             def rel(a, b):
                 return float(f"{(torch.linalg.vector_norm((a - b).flatten()) / torch.linalg.vector_norm(b.flatten())).item():.3g}")
@@ -681,7 +808,7 @@ def bench_vendor(torch, dev, stream, reps, with_torch=True):
                     i = cur[0] = (cur[0] + 1) % copies
                     outs[i] = None
                     outs[i] = tf(xs[i])
-                ms_torch = event_time_ms(torch, stream, via_torch, reps)
+                ms_torch = median_ms(torch, stream, via_torch, reps)
                 ours()
                 i = cur[0]
                 entry.update({"torch_fft_us": round(ms_torch * 1e3, 2), "speedup_vs_torch_fft": round(ms_torch / ms_ours, 3),
@@ -698,7 +825,7 @@ def bench_vendor(torch, dev, stream, reps, with_torch=True):
             def via_hipfft():
                 i = cur[0] = (cur[0] + 1) % copies
                 hip_exec(xs[i].data_ptr(), ys[i].data_ptr())
-            ms_hip = event_time_ms(torch, stream, via_hipfft, reps)
+            ms_hip = median_ms(torch, stream, via_hipfft, reps)
             # agreement on fresh copies of the input (rocFFT's C2R overwrites its input)
             xs[0].copy_(pristine)
             xs[1 % copies].copy_(pristine)
@@ -759,9 +886,9 @@ def bench_shift(torch, dev, stream, reps):
                         L.asdFftExecC2C(plain.handle, tt1.ptr, tt2.ptr) != 0 or
                         L.asdFftShift(X, Y, B, 0, tt2.ptr, ty.ptr, ctypes.c_void_p(sh), None) != 0):
                     raise RuntimeError("three-call sequence failed")
-            ms_f = event_time_ms(torch, stream, run_fused, reps)
+            ms_f = median_ms(torch, stream, run_fused, reps)
             yf = y.clone()
-            ms_3 = event_time_ms(torch, stream, run_three, reps)
+            ms_3 = median_ms(torch, stream, run_three, reps)
             torch.cuda.synchronize()
             rel = (torch.linalg.vector_norm(yf - y) / torch.linalg.vector_norm(y)).item()
             for t in (tx, ty, tt1, tt2):
@@ -803,8 +930,8 @@ def bench_istft(torch, dev, stream, reps):
                 st = L.asdFftExecIstft(p.handle, *[t.ptr for t in ts])
                 if st != 0:
                     raise RuntimeError(f"asdFftExecIstft returned {st}")
-            ms = event_time_ms(torch, stream, run, reps)
-            ms_torch = event_time_ms(torch, stream, lambda: torch.istft(
+            ms = median_ms(torch, stream, run, reps)
+            ms_torch = median_ms(torch, stream, lambda: torch.istft(
                 spec, n_fft=nfft, hop_length=hop, win_length=nfft, window=win, center=True, normalized=False,
                 onesided=False, return_complex=True), reps)
             ref = torch.istft(spec, n_fft=nfft, hop_length=hop, win_length=nfft, window=win, center=True,
@@ -828,7 +955,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--kernel-reps", type=int, default=20, help="event-timed repetitions per kernel launch")
+    ap.add_argument("--samples", type=int, default=300,
+                    help="per-sample event timings per workload and per launch (median / mean reported)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--only", default=None, help="run one workload entry by name (profiling)")
     ap.add_argument("--verbose", action="store_true")
@@ -862,22 +990,22 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     if a.ragged_only:
-        print(json.dumps({"forms": {"ragged": bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)}}))
+        print(json.dumps({"forms": {"ragged": bench_ragged(torch, dev, stream, max(5, a.samples), 3)}}))
         return
     if a.serve_only:
         print(json.dumps({"forms": {"serving": bench_serve(torch, dev)}}))
         return
     if a.separated_only:
-        print(json.dumps({"forms": {"separated": bench_separated(torch, dev, stream, max(5, a.kernel_reps))}}))
+        print(json.dumps({"forms": {"separated": bench_separated(torch, dev, stream, max(5, a.samples))}}))
         return
     if a.shift_only:
-        print(json.dumps({"forms": {"shift": bench_shift(torch, dev, stream, max(5, a.kernel_reps))}}))
+        print(json.dumps({"forms": {"shift": bench_shift(torch, dev, stream, max(5, a.samples))}}))
         return
     if a.istft_only:
-        print(json.dumps({"forms": {"istft": bench_istft(torch, dev, stream, max(5, a.kernel_reps))}}))
+        print(json.dumps({"forms": {"istft": bench_istft(torch, dev, stream, max(5, a.samples))}}))
         return
     if a.vendor_only:
-        print(json.dumps({"forms": {"vendor": bench_vendor(torch, dev, stream, max(5, a.kernel_reps),
+        print(json.dumps({"forms": {"vendor": bench_vendor(torch, dev, stream, max(5, a.samples),
                                                            not a.vendor_no_torch)}}))
         return
     L = N.lib()
@@ -900,7 +1028,7 @@ def main():
             tin = N.AclTensor(x.shape, N.ACL_FLOAT if x.dtype == torch.float32 else N.ACL_COMPLEX64, x.data_ptr())
             tout = N.AclTensor(y.shape, N.ACL_FLOAT if y.dtype == torch.float32 else N.ACL_COMPLEX64, y.data_ptr())
             fn = {"c2c": L.asdFftExecC2C, "r2c": L.asdFftExecR2C, "c2r": L.asdFftExecC2R}[kind]
-            jobs.append(dict(name=name, kind=kind, dims=dims, batch=batch, plan=plan, x=x, y=y, tin=tin,
+            jobs.append(dict(name=name, kind=kind, dims=dims, batch=batch, inv=inv, plan=plan, x=x, y=y, tin=tin,
                              tout=tout, fn=fn, flop=flops(dims, batch), bytes=alg_bytes(kind, dims, batch),
                              steps=plan.step_bytes(), describe=plan.describe()))
             prev_out[name] = y
@@ -953,24 +1081,42 @@ def main():
                 st = j["fn"](j["plan"].handle, ti.ptr, to.ptr)
                 if st != 0:
                     raise RuntimeError(f"{j['name']}: exec returned {st}")
-            ms = event_time_ms(torch, stream, run_ring, a.kernel_reps)
-            breakdown.append({"name": j["name"], "us": round(ms * 1e3, 2),
-                              "gflops": round(j["flop"] / ms / 1e6, 1),
-                              "alg_GBps": round(j["bytes"] / ms / 1e6, 1),
-                              "hbm_frac": round(j["bytes"] / ms / 1e6 / HBM_PEAK_GBS, 3)})
+            dt = device_times_us(torch, stream, run_ring, a.samples)
+            ms = dt["median"] * 1e-3
+            ms_b2b = event_time_ms(torch, stream, run_ring, a.samples)
+            # copy ceiling: a device-to-device copy of the same bytes (read input-size, write
+            # output-size, the larger side split so both sides move) over the same ring
+            cpy = copy_ceiling_us(torch, stream, j["x"], j["y"], a.samples)
+            entry = {"name": j["name"], "us": round(ms * 1e3, 2), "us_p10": round(dt["p10"], 2),
+                     "us_p90": round(dt["p90"], 2), "samples": dt["n"],
+                     "us_back_to_back": round(ms_b2b * 1e3, 2),
+                     "gflops": round(j["flop"] / ms / 1e6, 1),
+                     "alg_GBps": round(j["bytes"] / ms / 1e6, 1),
+                     "hbm_frac": round(j["bytes"] / ms / 1e6 / HBM_PEAK_GBS, 3),
+                     "copy_us": round(cpy, 2), "frac_of_copy": round(cpy / (ms * 1e3), 3)}
+            if j["kind"] in ("r2c", "c2r"):
+                entry["gflops_conventional_2.5NlogN"] = round(0.5 * j["flop"] / ms / 1e6, 1)
+            breakdown.append(entry)
             for si, sb in enumerate(j["steps"]):
                 def one(si=si):
                     ti, to = next_pair()
                     st = L.asdFftExtExecStep(j["plan"].handle, ti.ptr, to.ptr, si)
                     if st != 0:
                         raise RuntimeError(f"{j['name']} step {si}: {st}")
-                kms = event_time_ms(torch, stream, one, a.kernel_reps)
-                launches.append({"workload": j["name"], "step": si, "us": kms * 1e3, "bytes": sb,
-                                 "GBps": sb / kms / 1e6, "ring_pairs": len(ring)})
+                kt = device_times_us(torch, stream, one, a.samples)
+                kms = kt["mean"] * 1e-3
+                launches.append({"workload": j["name"], "step": si, "us": kms * 1e3, "us_median": kt["median"],
+                                 "bytes": sb, "GBps": sb / kms / 1e6, "ring_pairs": len(ring)})
             for ti, to in ring[1:]:
                 ti.close()
                 to.close()
             del keep
+    # per-configuration error against a float64 FFT, outside the timed region
+    for j, entry in zip(jobs, breakdown):
+        with torch.cuda.stream(stream):
+            run_job(j)
+        torch.cuda.synchronize()
+        entry["rel_l2"] = sampled_error(torch, j["kind"], j["dims"], j["batch"], j["inv"], j["x"], j["y"])
     total_flop = sum(j["flop"] for j in jobs)
     value = total_flop / ms_per_step / 1e6  # GFLOP/s
 
@@ -1019,24 +1165,26 @@ def main():
         del src, dst
     forms = {}
     if not a.no_ragged:
-        forms["ragged"] = bench_ragged(torch, dev, stream, max(5, a.kernel_reps), 3)
+        forms["ragged"] = bench_ragged(torch, dev, stream, max(5, a.samples), 3)
     if not a.no_serve:
         forms["serving"] = bench_serve(torch, dev)
     if not a.no_precision:
-        forms["precision"] = bench_precision(torch, dev, stream, max(5, a.kernel_reps))
+        forms["precision"] = bench_precision(torch, dev, stream, max(5, a.samples))
     if not a.no_separated:
-        forms["separated"] = bench_separated(torch, dev, stream, max(5, a.kernel_reps))
+        forms["separated"] = bench_separated(torch, dev, stream, max(5, a.samples))
     if not a.no_istft:
-        forms["istft"] = bench_istft(torch, dev, stream, max(5, a.kernel_reps))
+        forms["istft"] = bench_istft(torch, dev, stream, max(5, a.samples))
     if not a.no_shift:
-        forms["shift"] = bench_shift(torch, dev, stream, max(5, a.kernel_reps))
+        forms["shift"] = bench_shift(torch, dev, stream, max(5, a.samples))
     if not a.no_vendor:
-        forms["vendor"] = bench_vendor(torch, dev, stream, max(5, a.kernel_reps))
+        forms["vendor"] = bench_vendor(torch, dev, stream, max(5, a.samples))
     if forms:
         line["forms"] = forms
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
-        forms["cpu_numpy"] = cpu_numpy()
+        numpy_cpu = cpu_numpy()
+        line["cpu_baseline"] = numpy_cpu.pop("baseline")
+        forms["cpu_numpy"] = numpy_cpu
+        forms["cpu_baseline_oracle"] = cpu_oracle()
         line["forms"] = forms
     for j in jobs:
         j["tin"].close()

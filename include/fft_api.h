@@ -95,15 +95,16 @@ AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, asdFftType fft
                             int64_t batchSize, asdFft1dDimType dimType ASDFFT_DEFAULT(ASCEND_FFT_HORIZONTAL));
 
 /* ISTFT plan for a complex64 (channel, nFft, nFrames) spectrum: center = true, normalized = false,
- * onesided = false, lengthOpt = 0, returnComplex = true (other values are replaced with a warning,
- * as the reference does).  Requires nFrames > 1, 0 < hopLengthOpt <= winLengthOpt == nFft,
+ * onesided = false, lengthOpt = 0, returnComplex = true, the reference's defaults (other values are
+ * replaced with a warning, as the reference does).  Requires nFrames > 1, 0 < hopLengthOpt <= winLengthOpt == nFft,
  * nFft < 1500, hopLengthOpt < 1500 (else ACL_ERROR_INVALID_PARAM); an initialised handle ->
  * ACL_ERROR_INTERNAL_ERROR.  The workspace holds the inverse-transformed frames.
  * Replaces include/fft_api.h:59-62 (core/fft/istft.cpp:343-433, 525-570). */
 AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor* input, const int64_t nFft,
-                               const int64_t hopLengthOpt, const int64_t winLengthOpt, const bool center,
-                               const bool normalized, const bool onesidedOpt, int64_t lengthOpt,
-                               const bool returnComplex);
+                               const int64_t hopLengthOpt, const int64_t winLengthOpt,
+                               const bool center ASDFFT_DEFAULT(true), const bool normalized ASDFFT_DEFAULT(false),
+                               const bool onesidedOpt ASDFFT_DEFAULT(false), int64_t lengthOpt ASDFFT_DEFAULT(0),
+                               const bool returnComplex ASDFFT_DEFAULT(true));
 
 /* Replaces include/fft_api.h:65 (fft_api.cpp:1752-1788). */
 AspbStatus asdFftExecC2C(asdFftHandle handle, const aclTensor* input, const aclTensor* output);
@@ -157,11 +158,16 @@ AspbStatus asdFftSynchronize(asdFftHandle handle);
 /* Replaces include/fft_api.h:95 (fft_api.cpp:1392-1404). */
 AspbStatus asdFftGetType(asdFftHandle handle, ASDFFT_OUT(asdFftType) fftType);
 
-/* Not part of this engine's path: return ACL_ERROR_API_NOT_SUPPORT.  Declared by include/fft_api.h:105-122. */
+/* Batched 2D circular shift of complex64 slices on `stream` (a hipStream_t):
+ *   asdFftShift : out[s][(r + ceil(row/2)) % row][(c + ceil(col/2)) % col] = in[s*dist + r*col + c]
+ *   asdIfftShift: out[s][(r + row/2) % row][(c + col/2) % col] = float(scale) * in[s*dist + r*col + c]
+ * dist <= 0 means contiguous slices (dist = row * col); the output is a contiguous
+ * (batch, row, col) tensor.  workspace is unused (the reference takes none either).
+ * Replaces include/fft_api.h:105-106 and :121-122 (core/fft/fftshift.cpp:101-138, ifftshift.cpp). */
 AspbStatus asdFftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, const aclTensor* input,
-                       aclTensor* output, void* stream, void* workspace);
+                       aclTensor* output, void* stream, void* workspace ASDFFT_DEFAULT(nullptr));
 AspbStatus asdIfftShift(int64_t row, int64_t col, int64_t batch, int64_t dist, double scale, const aclTensor* input,
-                        aclTensor* output, void* stream, void* workspace);
+                        aclTensor* output, void* stream, void* workspace ASDFFT_DEFAULT(nullptr));
 
 #ifdef __cplusplus
 }  // namespace AsdSip

@@ -137,6 +137,44 @@ def build_torch_ops(force: bool = False) -> str:
     return TORCH_LIB
 
 
+def host_driver(driver: str, out: str, sanitize: str | None = None, jobs: int | None = None) -> str:
+    """Links a host program (driver) against the library's host sources compiled for it,
+    optionally instrumented (sanitize: "address,undefined" or "thread"), and the device
+    objects of the last build().  The host sources are recompiled for each instrumentation
+    into their own folder under _build; nothing of libasdfft.so changes."""
+    dev_objs = [os.path.join(BUILD, f"stockham_v{v}_e{e}.o") for v, e in instances()]
+    static_hips, reg_src = generate_static_sources()
+    dev_objs += [src[:-4] + ".o" for src in static_hips]
+    dev_objs += [os.path.join(BUILD, f"ragged_kernel_e{e}.o") for e in RAGGED_EMAX]
+    dev_objs += [os.path.join(BUILD, f.replace(".hip", ".o")) for f in KERNEL_SOURCES]
+    if not all(os.path.exists(o) for o in dev_objs):
+        build(jobs=jobs)
+    tag = (sanitize or "plain").replace(",", "_")
+    odir = os.path.join(BUILD, f"host_{tag}")
+    os.makedirs(odir, exist_ok=True)
+    flags = ["-O1", "-g", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{INCLUDE}",
+             f"-I{CSRC}", "-fno-omit-frame-pointer"]
+    if sanitize:
+        flags += [f"-fsanitize={sanitize}", "-fno-sanitize-recover=all"]
+    host_hdr = [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    host_hdr += [os.path.join(INCLUDE, h) for h in os.listdir(INCLUDE) if h.endswith(".h")]
+    tasks, objs = [], []
+    for src in [os.path.join(CSRC, f) for f in HOST_SOURCES] + [reg_src, driver]:
+        obj = os.path.join(odir, os.path.basename(src).rsplit(".", 1)[0] + ".o")
+        objs.append(obj)
+        if _newer(obj, [src] + host_hdr):
+            tasks.append(["g++"] + flags + ["-c", src, "-o", obj])
+    with ThreadPoolExecutor(max_workers=jobs or min(16, os.cpu_count() or 4)) as ex:
+        list(ex.map(_run, tasks))
+    if tasks or _newer(out, objs + dev_objs):
+        link = ["g++", "-o", out] + objs + dev_objs + [f"-L{ROCM}/lib", "-lamdhip64", "-lpthread",
+                                                       f"-Wl,-rpath,{ROCM}/lib"]
+        if sanitize:
+            link.insert(1, f"-fsanitize={sanitize}")
+        _run(link)
+    return out
+
+
 def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)

@@ -83,8 +83,8 @@ hipError_t launch_dft_direct(const LaunchShape& s, const DftArgs& args, hipStrea
 {
     if (s.grid <= 0 || s.block <= 0 || s.block > 256) return hipErrorInvalidConfiguration;
     if (s.lds_bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(dft_direct_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
+        static std::atomic<uint64_t> raised{0};
+        hipError_t e = raise_lds_limit_once(reinterpret_cast<const void*>(dft_direct_kernel), raised);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(dft_direct_kernel, dim3((unsigned)s.grid), dim3((unsigned)s.block), s.lds_bytes, stream, args);

@@ -64,7 +64,11 @@ struct FftPlan {
     bool shifted() const { return in_shift != asdfft::kShiftNone || out_shift != asdfft::kShiftNone || out_scale != 1.0f; }
 };
 
+// Handles are keys of this table, drawn from a counter that never repeats a value, not plan
+// addresses: a stale handle (used after asdFftDestroy) then never names a plan created later,
+// so it is rejected instead of acting on another caller's plan.
 std::mutex g_table_mu;
+uintptr_t g_next_key = 0x1000;  // guarded by g_table_mu
 std::unordered_map<void*, std::shared_ptr<FftPlan>>& table()
 {
     static auto* t = new std::unordered_map<void*, std::shared_ptr<FftPlan>>();
@@ -463,8 +467,9 @@ AspbStatus asdFftCreate(asdFftHandle& handle)  // fft_api.cpp:1047-1065
 {
     try {
         auto p = std::make_shared<FftPlan>();
-        void* key = p.get();
         std::lock_guard<std::mutex> lock(g_table_mu);
+        g_next_key += 0x10;
+        void* key = reinterpret_cast<void*>(g_next_key);
         table()[key] = std::move(p);
         handle = key;
     } catch (const std::exception& e) {

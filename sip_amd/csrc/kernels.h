@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+
+#include <atomic>
 #include <stdint.h>
 
 #include "fft_kernel_args.h"
@@ -40,6 +42,21 @@ enum Variant : int {
 // workgroup.  The limit only admits launches, so it is never lowered to a launch's own size
 // (launches of different sizes from several threads could otherwise find it lowered).
 constexpr int kMaxDynamicLds = 160 * 1024;
+
+// Raises fn's dynamic-LDS limit to kMaxDynamicLds the first time it is launched on the
+// current device; `raised` (one per kernel) keeps one bit per device, so later launches make
+// no attribute call.
+inline hipError_t raise_lds_limit_once(const void* fn, std::atomic<uint64_t>& raised)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = uint64_t(1) << (dev & 63);
+    if (raised.load(std::memory_order_acquire) & bit) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
+    if (e == hipSuccess) raised.fetch_or(bit, std::memory_order_release);
+    return e;
+}
 
 struct LaunchShape {
     int static_id = -1;  // index into kStaticInstances, or -1 for the run-time kernel

@@ -395,6 +395,12 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     os << " emax=" << c.emax << " T=" << c.threads << " tpb=" << c.tpb << " block=" << st->shape.block
        << " lds=" << c.lds << " grid=" << st->shape.grid;
     if (st->shape.static_id >= 0) os << " static=" << st->shape.static_id;
+    // element strides and the largest element offset within one transform; the kernels form
+    // offsets in 64 bits, so these may exceed 2^31
+    if (io.in_stride != 1 || io.out_stride != 1) {
+        os << " stride=" << io.in_stride << "/" << io.out_stride << " span=" << (n - 1) * io.in_stride << "/"
+           << (n - 1) * io.out_stride;
+    }
     st->describe = os.str();
     return true;
 }

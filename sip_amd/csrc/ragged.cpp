@@ -503,6 +503,69 @@ hipError_t node_params(const RaggedPlan& p, size_t li, RaggedArgs* args, void** 
     return hipSuccess;
 }
 
+// Helper streams and events of the side-by-side launches, created all together or not at
+// all: a failure part-way destroys what was made, so the next exec starts from nothing.
+hipError_t ensure_helpers(RaggedPlan& p, size_t nl)
+{
+    if (nl <= 1 || !p.aux.empty()) return hipSuccess;
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i + 1 < nl && e == hipSuccess; i++) {
+        hipStream_t st;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) break;
+        aux.push_back(st);
+        hipEvent_t ev;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) break;
+        joins.push_back(ev);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        for (hipStream_t st : aux) (void)hipStreamDestroy(st);
+        for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
+        return e;
+    }
+    p.aux = std::move(aux);
+    p.joins = std::move(joins);
+    p.fork = fork;
+    return hipSuccess;
+}
+
+// Builds and instantiates the graph of the launches, or leaves no graph, nodes or exec
+// behind when any step fails.
+hipError_t build_graph(RaggedPlan& p, const void* in, void* out)
+{
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipGraphCreate(&graph, 0);
+    if (e != hipSuccess) return e;
+    std::vector<hipGraphNode_t> nodes;
+    // the launches touch disjoint outputs: independent nodes
+    for (size_t li = 0; li < p.launches.size() && e == hipSuccess; li++) {
+        RaggedArgs a = args_for(p, li, in, out);
+        void* slots[3];
+        hipKernelNodeParams kp;
+        e = node_params(p, li, &a, slots, &kp);
+        hipGraphNode_t node;
+        if (e == hipSuccess) e = hipGraphAddKernelNode(&node, graph, nullptr, 0, &kp);
+        if (e == hipSuccess) nodes.push_back(node);
+    }
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return e;
+    }
+    p.graph = graph;
+    p.nodes = std::move(nodes);
+    p.graph_exec = exec;
+    p.graph_in = in;
+    p.graph_out = out;
+    return hipSuccess;
+}
+
 }  // namespace
 
 hipError_t exec_ragged(RaggedPlan& p, const void* in, void* out, hipStream_t stream)
@@ -510,20 +573,10 @@ hipError_t exec_ragged(RaggedPlan& p, const void* in, void* out, hipStream_t str
     if (!p.ready) return hipErrorNotReady;
     if (!p.opt.graph) {
         const size_t nl = p.launches.size();
-        if (nl > 1 && p.aux.empty()) {
-            // the register budgets' launches run side by side on helper streams: a budget
-            // with few tiles (long transforms) would otherwise leave most CUs idle
-            for (size_t i = 0; i + 1 < nl; i++) {
-                hipStream_t st;
-                hipEvent_t ev;
-                if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return hipErrorOutOfMemory;
-                p.aux.push_back(st);
-                if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return hipErrorOutOfMemory;
-                p.joins.push_back(ev);
-            }
-            if (hipEventCreateWithFlags(&p.fork, hipEventDisableTiming) != hipSuccess) return hipErrorOutOfMemory;
-        }
-        hipError_t e;
+        // the register budgets' launches run side by side on helper streams: a budget
+        // with few tiles (long transforms) would otherwise leave most CUs idle
+        hipError_t e = ensure_helpers(p, nl);
+        if (e != hipSuccess) return e;
         if (nl > 1) {
             e = hipEventRecord(p.fork, stream);
             if (e != hipSuccess) return e;
@@ -552,32 +605,15 @@ hipError_t exec_ragged(RaggedPlan& p, const void* in, void* out, hipStream_t str
     }
     hipError_t e;
     if (p.graph_exec == nullptr) {
-        e = hipGraphCreate(&p.graph, 0);
+        e = build_graph(p, in, out);
         if (e != hipSuccess) return e;
-        // the launches touch disjoint outputs: independent nodes
-        for (size_t li = 0; li < p.launches.size(); li++) {
-            RaggedArgs a = args_for(p, li, in, out);
-            void* slots[3];
-            hipKernelNodeParams kp;
-            node_params(p, li, &a, slots, &kp);
-            hipGraphNode_t node;
-            e = hipGraphAddKernelNode(&node, p.graph, nullptr, 0, &kp);
-            if (e != hipSuccess) return e;
-            p.nodes.push_back(node);
-        }
-        e = hipGraphInstantiate(&p.graph_exec, p.graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            p.graph_exec = nullptr;
-            return e;
-        }
-        p.graph_in = in;
-        p.graph_out = out;
     } else if (in != p.graph_in || out != p.graph_out) {
         for (size_t li = 0; li < p.launches.size(); li++) {
             RaggedArgs a = args_for(p, li, in, out);
             void* slots[3];
             hipKernelNodeParams kp;
-            node_params(p, li, &a, slots, &kp);
+            e = node_params(p, li, &a, slots, &kp);
+            if (e != hipSuccess) return e;
             e = hipGraphExecKernelNodeSetParams(p.graph_exec, p.nodes[li], &kp);
             if (e != hipSuccess) return e;
         }

@@ -712,11 +712,21 @@ int serve_replay(Server* s, const ServeRequest* reqs, const double* arrival_us, 
             continue;
         }
         std::unique_lock<std::mutex> lk(s->mu);
-        s->cv.wait(lk, [&] { return s->reqs[tickets[i]].status >= 0; });
-        Req& q = s->reqs[tickets[i]];
+        // a ticket that another thread's asdFftExtServeWait already retired is gone from the
+        // table: it counts as done (its latency went to that caller), never as a new entry
+        s->cv.wait(lk, [&] {
+            auto it = s->reqs.find(tickets[i]);
+            return it == s->reqs.end() || it->second.status >= 0;
+        });
+        auto it = s->reqs.find(tickets[i]);
+        if (it == s->reqs.end()) {
+            latency_us[i] = -1.0;
+            continue;
+        }
+        Req& q = it->second;
         latency_us[i] = std::chrono::duration<double, std::micro>(q.completed - due[i]).count();
         if (q.status != 0) worst = std::max(worst, q.status);
-        s->reqs.erase(tickets[i]);
+        s->reqs.erase(it);
     }
     return worst;
 }

@@ -283,7 +283,8 @@ __device__ __forceinline__ float2 conj_if(float2 x, uint32_t mask)
 }
 
 // ------------------------------------------------------------------ prologue / epilogue
-// Per-thread view of one transform: base pointers plus 32-bit element strides.
+// Per-thread view of one transform: base pointers plus 64-bit element strides (a column
+// pass of a 3D plan has stride Y*Z, and (n - 1) * stride can pass 2^31 elements).
 struct Io {
     const void* in;   // first element of this transform's input
     void* out;        // first element of this transform's output
@@ -292,12 +293,12 @@ struct Io {
     int64_t col;      // column index (four-step twiddle)
     uint32_t cin;     // sign mask applied to inputs (conj_in)
     uint32_t cout;    // sign mask applied to outputs (conj_out)
-    int in_stride;    // element stride (1 for rows)
-    int out_stride;
+    int64_t in_stride;    // element stride (1 for rows)
+    int64_t out_stride;
 };
 
 template <int MAP>
-__device__ __forceinline__ int stride_of(int s)
+__device__ __forceinline__ int64_t stride_of(int64_t s)
 {
     if constexpr (MAP == kMapRows) return 1;
     return s;
@@ -307,7 +308,7 @@ template <int MAP, int PRO>
 __device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int n, int i)
 {
     const float2* in2 = reinterpret_cast<const float2*>(io.in);
-    const int s = stride_of<MAP>(io.in_stride);
+    const int64_t s = stride_of<MAP>(io.in_stride);
     if constexpr (PRO == kProComplex) {
         return in2[i * s];  // conjugation (conj_in) is applied by the caller
     } else if constexpr (PRO == kProReal) {
@@ -350,7 +351,7 @@ template <int MAP, int EPI>
 __device__ __forceinline__ void store_output(const FftArgs& a, const Io& io, int n, int i, float2 x)
 {
     float2* out2 = reinterpret_cast<float2*>(io.out);
-    const int s = stride_of<MAP>(io.out_stride);
+    const int64_t s = stride_of<MAP>(io.out_stride);
     if constexpr (EPI == kEpiComplex) {
         out2[i * s] = x;  // conjugation (conj_out) is applied by the caller
     } else if constexpr (EPI == kEpiHalf) {
@@ -560,8 +561,8 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
     io.col = 0;
     io.cin = a.conj_in ? 0x80000000u : 0u;
     io.cout = a.conj_out ? 0x80000000u : 0u;
-    io.in_stride = (int)a.in_stride;
-    io.out_stride = (int)a.out_stride;
+    io.in_stride = a.in_stride;
+    io.out_stride = a.out_stride;
     if (live) {
         io.col = t % a.cols;
         const int64_t o = t / a.cols;
@@ -644,7 +645,7 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
         if (t2 < a.count) {
             const int64_t c2 = t2 % a.cols;
             const int64_t o = c2 * a.out_inner + (t2 / a.cols) * a.out_outer;
-            const int s = (int)a.out_stride;
+            const int64_t s = a.out_stride;
             if constexpr (EPI == kEpiTranspose) {
                 float2* ob = reinterpret_cast<float2*>(a.out) + o;
                 for (int i = lane / g.tpb; i < g.n; i += step) {
@@ -821,7 +822,7 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
     float* lp_t = (MAP == kMapRows) ? lp + tt * kTstride : lp + tt;
     if constexpr (P == 0) {
         const float2* in2 = reinterpret_cast<const float2*>(io.in);
-        const int s = stride_of<MAP>(io.in_stride);
+        const int64_t s = stride_of<MAP>(io.in_stride);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
@@ -854,7 +855,7 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
             for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
         }
         float2* out2 = reinterpret_cast<float2*>(io.out);
-        const int s = stride_of<MAP>(io.out_stride);
+        const int64_t s = stride_of<MAP>(io.out_stride);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = tid + b * T;
@@ -954,15 +955,16 @@ template <> struct VariantTraits<kVarRowsShiftBoth> { static constexpr int map =
 template <> struct VariantTraits<kVarColsShiftOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiShift; };
 template <> struct VariantTraits<kVarRowsPair2> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiPairRadix2; };
 
-template <typename K>
-hipError_t launch_kernel(K kern, const LaunchShape& s, const FftArgs& args, hipStream_t stream)
+// One instantiation per kernel, so the LDS limit is raised once per kernel and device.
+template <auto Kern>
+hipError_t launch_kernel(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     if (s.lds_bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
+        static std::atomic<uint64_t> raised{0};
+        hipError_t e = raise_lds_limit_once(reinterpret_cast<const void*>(Kern), raised);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3((unsigned)s.grid), dim3((unsigned)s.block), s.lds_bytes, stream, args);
+    hipLaunchKernelGGL(Kern, dim3((unsigned)s.grid), dim3((unsigned)s.block), s.lds_bytes, stream, args);
     return hipGetLastError();
 }
 
@@ -970,22 +972,22 @@ template <int EMAX, int V>
 hipError_t launch_variant(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     using Tr = VariantTraits<V>;
-    return launch_kernel(stockham_kernel<EMAX, Tr::map, Tr::pro, Tr::epi>, s, args, stream);
+    return launch_kernel<stockham_kernel<EMAX, Tr::map, Tr::pro, Tr::epi>>(s, args, stream);
 }
 
 template <int V, int EMAX, int N, int T, int TPB, int... Rs>
 hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t stream)
 {
     if constexpr (V == kVarColsC2CSplit) {
-        return launch_kernel(stockham_split_kernel<EMAX, kMapCols, kEpiComplex, N, T, TPB, Rs...>, s, args, stream);
+        return launch_kernel<stockham_split_kernel<EMAX, kMapCols, kEpiComplex, N, T, TPB, Rs...>>(s, args, stream);
     } else if constexpr (V == kVarColsShiftOutSplit) {
-        return launch_kernel(stockham_split_kernel<EMAX, kMapCols, kEpiShift, N, T, TPB, Rs...>, s, args, stream);
+        return launch_kernel<stockham_split_kernel<EMAX, kMapCols, kEpiShift, N, T, TPB, Rs...>>(s, args, stream);
     } else if constexpr (V == kVarRowsC2CSplit) {
-        return launch_kernel(stockham_split_kernel<EMAX, kMapRows, kEpiComplex, N, T, TPB, Rs...>, s, args, stream);
+        return launch_kernel<stockham_split_kernel<EMAX, kMapRows, kEpiComplex, N, T, TPB, Rs...>>(s, args, stream);
     } else {
         using Tr = VariantTraits<V>;
-        return launch_kernel(stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>, s, args,
-                             stream);
+        return launch_kernel<stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>>(s, args,
+                                                                                                       stream);
     }
 }
 

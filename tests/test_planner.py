@@ -1,9 +1,11 @@
 """Planner tests (CPU): radix schedules, launch shapes and workspace sizes for the
 benchmark configurations, read back through asdFftExtDescribePlan."""
+import ctypes
 import re
 
 import pytest
 
+from sip_amd import _native as N
 from sip_amd.fft import Plan
 
 
@@ -136,3 +138,34 @@ def test_large_real_lengths_plan():
         assert first in s[0] and last in s[-1], (kind, n, s)
         assert p.workspace_size == 2 * m * 2 * 8, (kind, n, p.workspace_size)
         p.close()
+
+
+# Column passes whose in-transform element offsets pass 2^31: (n - 1) * stride for the axis-0
+# pass of a 3D plan (stride Y * Z) and for VERTICAL 1D (stride = batch).  The kernels take
+# 64-bit strides and form every element offset as a 64-bit product, so such plans are accepted
+# and described with their full spans (nothing is allocated or launched here).
+@pytest.mark.parametrize("make,n,stride", [
+    (lambda L, h: L.asdFftMakePlan3D(h, 4096, 1024, 520, N.ASCEND_FFT_C2C, N.ASCEND_FFT_FORWARD, 1), 4096, 1024 * 520),
+    (lambda L, h: L.asdFftMakePlan1D(h, 2048, N.ASCEND_FFT_C2C, N.ASCEND_FFT_FORWARD, 1100000,
+                                     N.ASCEND_FFT_VERTICAL), 2048, 1100000),
+])
+def test_column_spans_beyond_int32(make, n, stride):
+    import re
+    L = N.lib()
+    h = ctypes.c_void_p()
+    assert L.asdFftCreate(ctypes.byref(h)) == 0
+    try:
+        assert make(L, h) == 0
+        buf = ctypes.create_string_buffer(1 << 14)
+        need = ctypes.c_size_t()
+        assert L.asdFftExtDescribePlan(h, buf, len(buf), ctypes.byref(need)) == 0
+        text = buf.value.decode()
+        spans = [(int(m.group(1)), int(m.group(2)), int(m.group(3)), int(m.group(4)))
+                 for m in re.finditer(r"n=(\d+) .*stride=(\d+)/(\d+) span=(\d+)/", text)]
+        hit = [sp for sp in spans if sp[0] == n]
+        assert hit, text
+        nn, s_in, s_out, span = hit[0]
+        assert s_in == stride and s_out == stride
+        assert span == (n - 1) * stride > 2 ** 31
+    finally:
+        L.asdFftDestroy(h)
