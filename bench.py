@@ -584,9 +584,11 @@ def bench_serve(torch, dev, rates=(5000, 20000, 50000, 100000, 200000), count=30
 
 def bench_precision(torch, dev, stream, reps):
     """Precision form: 1D C2C N=65536 batch 64 and N=1024 batch 8192, inputs uniform[-1,1]
-    scaled by 10^k (k uniform in {-3..3} per transform).  Times the FP32 kernels and, where a
-    kernel exists (N=1024), the split-FP16 MFMA kernel; reports the relative L2 error of each
-    against numpy's FP64 FFT on 64 sampled transforms."""
+    scaled by 10^k (k uniform in {-3..3} per transform).  Times the FP32 plan and the split-FP16
+    plan (N=1024: both radix-32 stages on MFMA; N=65536: the FP32 64-point column pass, then the
+    1024-point rows on MFMA with a transposed store) per sample over a ring of (input, output)
+    pairs larger than the device cache; reports the relative L2 error of each against numpy's
+    FP64 FFT on 64 sampled transforms."""
     import numpy as np
 
     from sip_amd.fft import Plan
@@ -613,7 +615,18 @@ def bench_precision(torch, dev, stream, reps):
                 y = torch.empty_like(x)
                 p.bind_workspace(dev)
                 p.bind_stream(stream.cuda_stream)
-                ms = median_ms(torch, stream, lambda: p.execute(x, y), reps)
+                copies = ring_copies(x, y)
+                xs_ring = [x] + [x.clone() for _ in range(copies - 1)]
+                ys_ring = [y] + [torch.empty_like(y) for _ in range(copies - 1)]
+                cur = [0]
+
+                def run_ring():
+                    i = cur[0] = (cur[0] + 1) % copies
+                    p.execute(xs_ring[i], ys_ring[i])
+                ms = median_ms(torch, stream, run_ring, reps)
+                p.execute(x, y)
+                torch.cuda.synchronize()
+                del xs_ring, ys_ring
                 got = y[torch.as_tensor(rows, device=dev)].cpu().numpy().astype(np.complex128)
                 err = np.linalg.norm(got - gold, axis=1) / np.linalg.norm(gold, axis=1)
                 entry[prec] = {"us": round(ms * 1e3, 2), "gflops": round(flops((n,), batch) / ms / 1e6, 1),

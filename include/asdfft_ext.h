@@ -78,9 +78,11 @@ int32_t asdFftExtRaggedInfo(void* handle, int64_t* in_extent, int64_t* out_exten
 /* ---- precision form ------------------------------------------------------------------
  * ASDFFT_PRECISION_SPLIT16 runs large-radix stages as matrix products on FP16 MFMA with
  * split hi + lo operands (a b ~= a_hi b_hi + a_hi b_lo + a_lo b_hi) and FP32 accumulation,
- * after a per-transform power-of-two scale.  A kernel exists for 1D C2C N = 1024 (two
- * radix-32 stages); plans of other shapes keep the FP32 kernels.  Typical relative L2
- * error vs FP64 is a few 1e-7 (the FP32 kernels: about 1e-7). */
+ * after a per-row power-of-two scale.  Kernels exist for 1D C2C N = 1024 (two radix-32
+ * stages) and for the 1024-point pass of long 1D C2C lengths N = 1024 * n2 (n2 a multiple of
+ * 16, N > 16384, e.g. 65536: an FP32 n2-point column pass with the W_N twiddle, then the
+ * 1024-point rows on MFMA with a transposed store); plans of other shapes keep the FP32
+ * kernels.  Typical relative L2 error vs FP64 is a few 1e-7 (the FP32 kernels: about 1e-7). */
 #define ASDFFT_PRECISION_FP32 0
 #define ASDFFT_PRECISION_SPLIT16 1
 

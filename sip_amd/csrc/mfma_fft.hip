@@ -15,9 +15,20 @@
 // MFMA lane layout (32x32x16, wave64): operand A holds A[i][k] at lane l for i = l % 32 and
 // k = 8 (l / 32) + j, j = 0..7; operand B holds B[k][n] for n = l % 32, same k; the 16
 // accumulators hold D[i][n] for n = l % 32, i = 8 (v / 4) + 4 (l / 32) + v % 4.
+//
+// Two kernels share the row transform:
+//  * mfma_fft1024_kernel: contiguous rows in and out (a 1D plan of length 1024).  Each wave
+//    loops over rows and loads its next row while it transforms the current one.
+//  * mfma_fft1024_tr_kernel: the second pass of a four-step transform whose output is stored
+//    transposed: row t = g * R + k (R rows per group) writes element m to
+//    out[g * R * 1024 + k + R * m].  A workgroup transforms a tile of 16 consecutive rows of
+//    one group in LDS (8 waves, two rows each) and then stores the tile as 128-byte runs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
+#include "kernels.h"
 #include "mfma_fft.h"
 
 namespace asdfft {
@@ -27,9 +38,19 @@ namespace {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f16acc __attribute__((ext_vector_type(16)));
 
-constexpr int kWaves = 4;              // waves per workgroup, one row in flight each
-constexpr int kPitch = 33;             // LDS row pitch (floats) of the stage-2 transpose
+constexpr int kWaves = 4;              // plain kernel: waves per workgroup, one row in flight each
+constexpr int kPitch = 33;             // plain kernel: LDS row pitch (floats) of the stage-2 transpose
 constexpr int kPlane = 32 * kPitch;    // one 32x32 FP32 plane
+constexpr int kTrWaves = 8;            // transposed kernel: waves per workgroup
+constexpr int kTrRows = 16;            // transposed kernel: rows per tile (two per wave)
+constexpr int kTrPitch = kTrRows + 1;  // float2 slots per element index m in the tile
+
+// Tile slot of element m of tile row r.  The pitch of 17 and the skew m >> 5 keep both
+// stage-2 transpose patterns (fixed k1 with n2 = lane, fixed n2 with k1 = lane; m = 32 k1 + n2)
+// on distinct banks, and leave the 16 rows of one m contiguous for the final store.
+__device__ __forceinline__ int tr_slot(int m, int r) { return m * kTrPitch + (m >> 5) + r; }
+constexpr int kTrSlots = 1023 * kTrPitch + 31 + kTrRows;  // one past the largest slot
+constexpr size_t kTrLdsBytes = (size_t)kTrSlots * 8 + 1024 * 8;  // tile + W_1024 table
 
 struct Split {
     half8 hi, lo;
@@ -71,20 +92,22 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-__global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFftArgs a)
+__device__ __forceinline__ void split_into(float re, float im, Split& xr, Split& xi, int j)
 {
-    __shared__ float2 tw[1024];              // W_1024^m
-    __shared__ float tr[kWaves][2 * kPlane];  // per-wave transpose buffer (re plane, im plane)
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
+    const _Float16 rh = hi_of(re), ih = hi_of(im);
+    xr.hi[j] = rh;
+    xr.lo[j] = lo_of(re, rh);
+    xi.hi[j] = ih;
+    xi.lo[j] = lo_of(im, ih);
+}
 
-    // A operands: F[i][k] for i = lane % 32, k = 16 c + 8 (lane / 32) + j
-    Split fr[2], fi[2];
+// A operands: F[i][k] = W_32^(i k) for i = lane % 32, k = 16 c + 8 (lane / 32) + j
+__device__ __forceinline__ void load_dft_matrix(const float2* w32, int lane, Split (&fr)[2], Split (&fi)[2])
+{
     for (int c = 0; c < 2; c++) {
         for (int j = 0; j < 8; j++) {
             const int k = 16 * c + 8 * (lane >> 5) + j;
-            const float2 w = a.w32[((lane & 31) * k) & 31];
+            const float2 w = w32[((lane & 31) * k) & 31];
             const _Float16 rh = hi_of(w.x), ih = hi_of(w.y);
             fr[c].hi[j] = rh;
             fr[c].lo[j] = lo_of(w.x, rh);
@@ -92,81 +115,153 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
             fi[c].lo[j] = lo_of(w.y, ih);
         }
     }
+}
+
+// Stage-1 B operand of one row: B[n1][n2] = x[32 n1 + n2], n2 = lane % 32 (256-byte runs).
+__device__ __forceinline__ void load_row(const float2* x, int lane, float2 (&v)[2][8])
+{
+    for (int c = 0; c < 2; c++) {
+        for (int j = 0; j < 8; j++) v[c][j] = x[32 * (16 * c + 8 * (lane >> 5) + j) + (lane & 31)];
+    }
+}
+
+// The 1024-point transform of one row held in v (stage-1 B layout).  xpose_put(k1, n2, re, im)
+// and xpose_get(k1, n2, &re, &im) store and fetch the twiddled stage-1 result through LDS
+// (wave-private locations); put(m, re, im) receives output element m.
+template <typename XPut, typename XGet, typename Put>
+__device__ __forceinline__ void row_fft(float2 (&v)[2][8], const Split (&fr)[2], const Split (&fi)[2],
+                                        const float2* tw, int lane, float csign_in, float csign_out, XPut xpose_put,
+                                        XGet xpose_get, Put put)
+{
+    float m = 0.0f;
+    for (int c = 0; c < 2; c++) {
+        for (int j = 0; j < 8; j++) m = fmaxf(m, fmaxf(fabsf(v[c][j].x), fabsf(v[c][j].y)));
+    }
+    m = wave_max(m);
+    // scale 2^(9 - e) with m in [2^e, 2^(e+1)): the largest component lands in [2^9, 2^10)
+    const int e = (m > 0.0f) ? ilogbf(m) : 0;
+    const float s = ldexpf(1.0f, 9 - e);
+    Split xr[2], xi[2];
+    for (int c = 0; c < 2; c++) {
+        for (int j = 0; j < 8; j++) split_into(v[c][j].x * s, v[c][j].y * s * csign_in, xr[c], xi[c], j);
+    }
+    // ---- stage 1
+    f16acc yr = {}, yi = {};
+    for (int c = 0; c < 2; c++) cmatmul(fr[c], fi[c], xr[c], xi[c], yr, yi);
+    // ---- twiddle (and 2^-5), then transpose through LDS
+    for (int q = 0; q < 16; q++) {
+        const int k1 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        const int n2 = lane & 31;
+        const float2 w = tw[(k1 * n2) & 1023];
+        const float re = yr[q] * w.x - yi[q] * w.y;
+        const float im = yr[q] * w.y + yi[q] * w.x;
+        xpose_put(k1, n2, re * 0.03125f, im * 0.03125f);
+    }
+    // this wave's transpose writes complete before its lanes read other lanes' values
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    // ---- stage 2 B operand: B[n2][k1] = Y'[k1][n2], k1 = lane % 32, n2 = 16 c + 8 (lane / 32) + j
+    for (int c = 0; c < 2; c++) {
+        for (int j = 0; j < 8; j++) {
+            float re, im;
+            xpose_get(lane & 31, 16 * c + 8 * (lane >> 5) + j, &re, &im);
+            split_into(re, im, xr[c], xi[c], j);
+        }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before the locations are reused
+    f16acc zr = {}, zi = {};
+    for (int c = 0; c < 2; c++) cmatmul(fr[c], fi[c], xr[c], xi[c], zr, zi);
+    // ---- output: Z^T[k2][k1] -> X[k1 + 32 k2], k1 = lane % 32
+    const float back = 32.0f / s;
+    for (int q = 0; q < 16; q++) {
+        const int k2 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        put((lane & 31) + 32 * k2, zr[q] * back, zi[q] * back * csign_out);
+    }
+}
+
+__global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFftArgs a)
+{
+    __shared__ float2 tw[1024];              // W_1024^m
+    __shared__ float tr[kWaves][2 * kPlane];  // per-wave transpose buffer (re plane, im plane)
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
+    Split fr[2], fi[2];
+    load_dft_matrix(a.w32, lane, fr, fi);
     __syncthreads();
     const float csign_in = a.conj_in ? -1.0f : 1.0f;
     const float csign_out = a.conj_out ? -1.0f : 1.0f;
     float* tre = tr[wave];
     float* tim = tr[wave] + kPlane;
+    auto xput = [&](int k1, int n2, float re, float im) {
+        tre[k1 * kPitch + n2] = re;
+        tim[k1 * kPitch + n2] = im;
+    };
+    auto xget = [&](int k1, int n2, float* re, float* im) {
+        *re = tre[k1 * kPitch + n2];
+        *im = tim[k1 * kPitch + n2];
+    };
     const int64_t rows_per_step = (int64_t)gridDim.x * kWaves;
-    for (int64_t row = (int64_t)blockIdx.x * kWaves + wave; row < a.count; row += rows_per_step) {
-        const float2* x = a.in + row * 1024;
-        // ---- load: B operand of stage 1, B[n1][n2] = x[32 n1 + n2], n2 = lane % 32
+    int64_t row = (int64_t)blockIdx.x * kWaves + wave;
+    float2 next[2][8];
+    if (row < a.count) load_row(a.in + row * 1024, lane, next);
+    for (; row < a.count; row += rows_per_step) {
         float2 v[2][8];
         for (int c = 0; c < 2; c++) {
-            for (int j = 0; j < 8; j++) {
-                const int n1 = 16 * c + 8 * (lane >> 5) + j;
-                float2 t = x[32 * n1 + (lane & 31)];
-                t.y *= csign_in;
-                v[c][j] = t;
-            }
+            for (int j = 0; j < 8; j++) v[c][j] = next[c][j];
         }
-        float m = 0.0f;
-        for (int c = 0; c < 2; c++) {
-            for (int j = 0; j < 8; j++) m = fmaxf(m, fmaxf(fabsf(v[c][j].x), fabsf(v[c][j].y)));
-        }
-        m = wave_max(m);
-        // scale 2^(9 - e) with m in [2^e, 2^(e+1)): the largest component lands in [2^9, 2^10)
-        const int e = (m > 0.0f) ? ilogbf(m) : 0;
-        const float s = ldexpf(1.0f, 9 - e);
-        Split xr[2], xi[2];
-        for (int c = 0; c < 2; c++) {
-            for (int j = 0; j < 8; j++) {
-                const float re = v[c][j].x * s, im = v[c][j].y * s;
-                const _Float16 rh = hi_of(re), ih = hi_of(im);
-                xr[c].hi[j] = rh;
-                xr[c].lo[j] = lo_of(re, rh);
-                xi[c].hi[j] = ih;
-                xi[c].lo[j] = lo_of(im, ih);
-            }
-        }
-        // ---- stage 1
-        f16acc yr = {}, yi = {};
-        for (int c = 0; c < 2; c++) cmatmul(fr[c], fi[c], xr[c], xi[c], yr, yi);
-        // ---- twiddle (and 2^-5), then transpose through LDS: tre/tim[k1][n2]
-        for (int q = 0; q < 16; q++) {
-            const int k1 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-            const int n2 = lane & 31;
-            const float2 w = tw[(k1 * n2) & 1023];
-            const float re = yr[q] * w.x - yi[q] * w.y;
-            const float im = yr[q] * w.y + yi[q] * w.x;
-            tre[k1 * kPitch + n2] = re * 0.03125f;
-            tim[k1 * kPitch + n2] = im * 0.03125f;
-        }
-        // this wave's transpose writes complete before its lanes read other lanes' values
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        // ---- stage 2 B operand: B[n2][k1] = Y'[k1][n2], k1 = lane % 32, n2 = 16 c + 8 (lane / 32) + j
-        for (int c = 0; c < 2; c++) {
-            for (int j = 0; j < 8; j++) {
-                const int n2 = 16 * c + 8 * (lane >> 5) + j;
-                const int at = (lane & 31) * kPitch + n2;
-                const float re = tre[at], im = tim[at];
-                const _Float16 rh = hi_of(re), ih = hi_of(im);
-                xr[c].hi[j] = rh;
-                xr[c].lo[j] = lo_of(re, rh);
-                xi[c].hi[j] = ih;
-                xi[c].lo[j] = lo_of(im, ih);
-            }
-        }
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before the next row's writes
-        f16acc zr = {}, zi = {};
-        for (int c = 0; c < 2; c++) cmatmul(fr[c], fi[c], xr[c], xi[c], zr, zi);
-        // ---- store: Z^T[k2][k1] -> X[k1 + 32 k2], k1 = lane % 32
+        // the next row's loads are in flight while this row is transformed
+        if (row + rows_per_step < a.count) load_row(a.in + (row + rows_per_step) * 1024, lane, next);
         float2* y = a.out + row * 1024;
-        const float back = 32.0f / s;
-        for (int q = 0; q < 16; q++) {
-            const int k2 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-            y[(lane & 31) + 32 * k2] = make_float2(zr[q] * back, zi[q] * back * csign_out);
+        row_fft(v, fr, fi, tw, lane, csign_in, csign_out, xput, xget,
+                [&](int m, float re, float im) { y[m] = make_float2(re, im); });
+    }
+}
+
+__global__ void __launch_bounds__(64 * kTrWaves) mfma_fft1024_tr_kernel(const MfmaFftArgs a)
+{
+    extern __shared__ float2 lds[];
+    float2* tile = lds;               // kTrSlots slots: tile rows through the stage-2 transpose and output
+    float2* tw = lds + kTrSlots;      // W_1024^m
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
+    Split fr[2], fi[2];
+    load_dft_matrix(a.w32, lane, fr, fi);
+    __syncthreads();
+    const float csign_in = a.conj_in ? -1.0f : 1.0f;
+    const float csign_out = a.conj_out ? -1.0f : 1.0f;
+    // tile: rows t0 .. t0 + 15 of group g (R = out_rows rows per group, R % 16 == 0)
+    const int64_t t0 = (int64_t)blockIdx.x * kTrRows;
+    const int64_t g = t0 / a.out_rows;
+    const int64_t k0 = t0 - g * a.out_rows;
+    const int r0 = 2 * wave;
+    float2 v0[2][8], v1[2][8];
+    if (t0 + r0 < a.count) load_row(a.in + (t0 + r0) * 1024, lane, v0);
+    if (t0 + r0 + 1 < a.count) load_row(a.in + (t0 + r0 + 1) * 1024, lane, v1);
+    for (int h = 0; h < 2; h++) {
+        const int r = r0 + h;
+        if (t0 + r >= a.count) break;  // uniform per wave
+        auto xput = [&](int k1, int n2, float re, float im) { tile[tr_slot(32 * k1 + n2, r)] = make_float2(re, im); };
+        auto xget = [&](int k1, int n2, float* re, float* im) {
+            const float2 x = tile[tr_slot(32 * k1 + n2, r)];
+            *re = x.x;
+            *im = x.y;
+        };
+        auto put = [&](int m, float re, float im) { tile[tr_slot(m, r)] = make_float2(re, im); };
+        if (h == 0) {
+            row_fft(v0, fr, fi, tw, lane, csign_in, csign_out, xput, xget, put);
+        } else {
+            row_fft(v1, fr, fi, tw, lane, csign_in, csign_out, xput, xget, put);
         }
+    }
+    __syncthreads();
+    // store: out[g * R * 1024 + k0 + r + R * m], 16 consecutive rows r per element m
+    const int live = (int)((a.count - t0 < kTrRows) ? a.count - t0 : kTrRows);
+    float2* out = a.out + g * a.out_rows * 1024 + k0;
+    for (int e = threadIdx.x; e < 1024 * kTrRows; e += 64 * kTrWaves) {
+        const int m = e / kTrRows;
+        const int r = e - m * kTrRows;
+        if (r < live) out[r + a.out_rows * (int64_t)m] = tile[tr_slot(m, r)];
     }
 }
 
@@ -175,10 +270,23 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
 hipError_t launch_mfma_fft1024(const MfmaFftArgs& args, int64_t grid, hipStream_t stream)
 {
     if (grid <= 0 || args.count <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mfma_fft1024_kernel, dim3((unsigned)grid), dim3(64 * kWaves), 0, stream, args);
+    if (!args.transposed) {
+        hipLaunchKernelGGL(mfma_fft1024_kernel, dim3((unsigned)grid), dim3(64 * kWaves), 0, stream, args);
+        return hipGetLastError();
+    }
+    // one tile of 16 rows per workgroup; tiles never straddle two groups of out_rows rows
+    if (args.out_rows <= 0 || args.out_rows % kTrRows != 0 || args.count % args.out_rows != 0 ||
+        grid != (args.count + kTrRows - 1) / kTrRows)
+        return hipErrorInvalidValue;
+    static std::atomic<uint64_t> raised{0};
+    hipError_t e = raise_lds_limit_once(reinterpret_cast<const void*>(mfma_fft1024_tr_kernel), raised);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mfma_fft1024_tr_kernel, dim3((unsigned)grid), dim3(64 * kTrWaves), kTrLdsBytes, stream,
+                       args);
     return hipGetLastError();
 }
 
 int mfma_fft1024_waves() { return kWaves; }
+int mfma_fft1024_tr_rows() { return kTrRows; }
 
 }  // namespace asdfft

@@ -533,6 +533,46 @@ bool c2c_cols(int64_t n, int64_t count, int64_t cols, int64_t outer, int64_t str
     return true;
 }
 
+// First pass of a four-step transform of length n = n1 * n2: n2-point columns at stride n1
+// (count * n1 of them) times W_n^(column * k), from src into the workspace in the same layout.
+bool fourstep_pass_a(int64_t n, int64_t n1, int64_t count, int src, size_t src_offset, size_t work_offset, bool planar,
+                     int conj_in, Step* out, std::string* why)
+{
+    const int64_t n2 = n / n1;
+    int shift = 0;
+    while (((int64_t)1 << shift) * ((int64_t)1 << shift) < n) shift++;
+    TablePtr lo = upload_table("biglo:" + std::to_string(n) + ":" + std::to_string(shift),
+                               roots_table(n, (int64_t)1 << shift));
+    const int64_t nhi = (n + ((int64_t)1 << shift) - 1) >> shift;
+    TablePtr hi = upload_table("bighi:" + std::to_string(n) + ":" + std::to_string(shift),
+                               roots_table(n, nhi, (int64_t)1 << shift));
+    Step& a = *out;
+    a.src = src;
+    a.src_offset = src_offset;
+    a.dst = kBufWork;
+    a.dst_offset = work_offset;
+    a.planar_in = planar;
+    a.bytes = 16.0 * (double)n * (double)count;
+    StepIo ia;
+    ia.count = count * n1;
+    ia.cols = n1;
+    ia.in_inner = 1;
+    ia.in_outer = n;
+    ia.in_stride = n1;
+    ia.out_inner = 1;
+    ia.out_outer = n;
+    ia.out_stride = n1;
+    if (!stockham_step(n2, planar ? kVarColsTwiddleSep : kVarColsTwiddle, true, ia, conj_in, 0, 0, &a, why)) return false;
+    a.args.tw_big_lo = lo ? lo->dev : nullptr;
+    a.args.tw_big_hi = hi ? hi->dev : nullptr;
+    a.args.big_shift = shift;
+    if (lo) a.tables.push_back(lo);
+    if (hi) a.tables.push_back(hi);
+    a.tables_ready = a.tables_ready && lo && hi;
+    a.describe += " (four-step A n=" + std::to_string(n) + ")";
+    return true;
+}
+
 // Large complex length: n = n1 * n2, pass A = length-n2 columns (stride n1) with the
 // W_n^{n1 k2} twiddle folded into its store, pass B = length-n1 rows stored transposed.
 // planar: pass A reads and pass B writes a pair of float planes (real, imaginary).
@@ -566,38 +606,8 @@ bool c2c_fourstep(int64_t n, int64_t count, bool inverse, int src, int dst, size
         return false;
     }
     const int64_t n1 = best1, n2 = n / n1;
-    int shift = 0;
-    while (((int64_t)1 << shift) * ((int64_t)1 << shift) < n) shift++;
-    TablePtr lo = upload_table("biglo:" + std::to_string(n) + ":" + std::to_string(shift),
-                               roots_table(n, (int64_t)1 << shift));
-    const int64_t nhi = (n + ((int64_t)1 << shift) - 1) >> shift;
-    TablePtr hi = upload_table("bighi:" + std::to_string(n) + ":" + std::to_string(shift),
-                               roots_table(n, nhi, (int64_t)1 << shift));
-
     Step a;
-    a.src = src;
-    a.src_offset = src_offset;
-    a.dst = kBufWork;
-    a.dst_offset = work_offset;
-    a.planar_in = planar;
-    a.bytes = 16.0 * (double)n * (double)count;
-    StepIo ia;
-    ia.count = count * n1;
-    ia.cols = n1;
-    ia.in_inner = 1;
-    ia.in_outer = n;
-    ia.in_stride = n1;
-    ia.out_inner = 1;
-    ia.out_outer = n;
-    ia.out_stride = n1;
-    if (!stockham_step(n2, planar ? kVarColsTwiddleSep : kVarColsTwiddle, true, ia, conj_in, 0, 0, &a, why)) return false;
-    a.args.tw_big_lo = lo ? lo->dev : nullptr;
-    a.args.tw_big_hi = hi ? hi->dev : nullptr;
-    a.args.big_shift = shift;
-    if (lo) a.tables.push_back(lo);
-    if (hi) a.tables.push_back(hi);
-    a.tables_ready = a.tables_ready && lo && hi;
-    a.describe += " (four-step A n=" + std::to_string(n) + ")";
+    if (!fourstep_pass_a(n, n1, count, src, src_offset, work_offset, planar, conj_in, &a, why)) return false;
 
     Step b;
     b.src = kBufWork;
@@ -784,6 +794,51 @@ void mfma_rows_1024(int64_t count, bool inverse, BuiltPlan* bp)
     os << "mfma split-fp16 n=1024 count=" << count << " stages=32x32 grid=" << st.mfma_grid;
     st.describe = os.str();
     bp->steps.push_back(std::move(st));
+}
+
+// Whether a long C2C length has the split-FP16 four-step: n = 1024 * n2 with n2 a whole number
+// of the transposed MFMA kernel's row tiles and an FP32 single-kernel column length.
+bool mfma_fourstep_fits(int64_t n)
+{
+    if (n <= kMaxSingleKernel || n % 1024 != 0) return false;
+    const int64_t n2 = n / 1024;
+    return n2 % mfma_fft1024_tr_rows() == 0 && n2 <= 4096 && smooth_single(n2);
+}
+
+// Long C2C, n = 1024 * n2, with the four-step's 1024-point rows on the split-FP16 MFMA kernel:
+// pass A is the FP32 n2-point column pass with the W_n twiddle (into the workspace), pass B
+// transforms the count * n2 rows of 1024 points (two radix-32 matmul stages) and stores
+// element j of row k of each transform to k + n2 j.
+bool mfma_fourstep(int64_t n, int64_t count, bool inverse, BuiltPlan* bp, std::string* why)
+{
+    const int64_t n1 = 1024, n2 = n / n1;
+    Step a;
+    if (!fourstep_pass_a(n, n1, count, kBufIn, 0, 0, false, inverse ? 1 : 0, &a, why)) return false;
+    Step b;
+    b.mfma = true;
+    b.src = kBufWork;
+    b.dst = kBufOut;
+    b.bytes = 16.0 * (double)n * (double)count;
+    TablePtr w32 = upload_table("roots:32", roots_table(32, 32));
+    TablePtr w1024 = upload_table("roots:1024", roots_table(1024, 1024));
+    b.tables_ready = w32 && w1024;
+    if (w32) b.tables.push_back(w32);
+    if (w1024) b.tables.push_back(w1024);
+    b.margs.w32 = w32 ? reinterpret_cast<const float2*>(w32->dev) : nullptr;
+    b.margs.tw1024 = w1024 ? reinterpret_cast<const float2*>(w1024->dev) : nullptr;
+    b.margs.count = count * n2;
+    b.margs.conj_in = 0;
+    b.margs.conj_out = inverse;
+    b.margs.transposed = 1;
+    b.margs.out_rows = n2;
+    b.mfma_grid = count * n2 / mfma_fft1024_tr_rows();
+    std::ostringstream os;
+    os << "mfma split-fp16 n=1024 count=" << count * n2 << " stages=32x32 transposed rows=" << n2
+       << " grid=" << b.mfma_grid << " (four-step B)";
+    b.describe = os.str();
+    bp->steps.push_back(std::move(a));
+    bp->steps.push_back(std::move(b));
+    return true;
 }
 
 }  // namespace
@@ -1062,6 +1117,11 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
         }
         if (spec.kind == Kind::C2C && spec.precision == Precision::kSplit16 && n == 1024) {
             mfma_rows_1024(B, inv, out);
+            return "";
+        }
+        if (spec.kind == Kind::C2C && spec.precision == Precision::kSplit16 && mfma_fourstep_fits(n)) {
+            out->workspace = (size_t)(B * n * 8);
+            if (!mfma_fourstep(n, B, inv, out, &why)) return why;
             return "";
         }
         if (spec.kind == Kind::C2C) {

@@ -59,7 +59,8 @@ enum Buffer : int { kBufIn = 0, kBufOut = 1, kBufWork = 2 };
 // Arithmetic of a plan's kernels.
 enum class Precision {
     kFp32 = 0,     // FP32 Stockham passes (default)
-    kSplit16 = 1,  // split-FP16 MFMA large-radix stages where a kernel exists (N = 1024 C2C rows)
+    kSplit16 = 1,  // split-FP16 MFMA large-radix stages where a kernel exists (C2C N = 1024 rows, and the
+                   // 1024-point pass of four-step lengths N = 1024 * n2, n2 a multiple of 16)
 };
 
 struct Step {

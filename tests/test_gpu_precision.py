@@ -1,8 +1,9 @@
 """GPU tests of the split-FP16 MFMA precision form (asdFftExtSetPrecision).
 
-Workload: 1D C2C N = 1024, inputs uniform[-1, 1] scaled by 10^k per transform with k
-uniform in {-3..3}.  Gate: relative L2 per transform vs the FP64 oracle <= 1e-6 * log2 N,
-the same bound as the FP32 kernels; both errors are printed side by side.
+Workload: 1D C2C N = 1024 and N = 65536 (the four-step whose 1024-point pass runs on MFMA;
+also 32768 and 131072), inputs uniform[-1, 1] scaled by 10^k per transform with k uniform in
+{-3..3}.  Gate: relative L2 per transform vs the FP64 oracle <= 1e-6 * log2 N, the same bound
+as the FP32 kernels; both errors are printed side by side.
 """
 import math
 
@@ -13,6 +14,7 @@ pytestmark = pytest.mark.gpu
 
 N = 1024
 TOL = 1e-6 * math.log2(N)
+MFMA_LENGTHS = (1024, 32768, 65536, 131072)  # lengths with a split-FP16 MFMA kernel
 
 
 @pytest.fixture(scope="module")
@@ -36,7 +38,7 @@ def run(torch_dev, x, inverse, precision, in_place=False):
     from sip_amd.fft import Plan
     p = Plan("c2c", (x.shape[1],), x.shape[0], inverse, precision=precision)
     try:
-        assert ("mfma split-fp16" in p.describe()) == (precision == "split16" and x.shape[1] == 1024)
+        assert ("mfma split-fp16" in p.describe()) == (precision == "split16" and x.shape[1] in MFMA_LENGTHS)
         xt = torch.from_numpy(x).to(torch_dev)
         out = p.execute(xt, out=xt if in_place else None)
         torch.cuda.synchronize()
@@ -92,3 +94,45 @@ def test_split16_unavailable_length_keeps_fp32(torch_dev, oracle):
     x = scaled_input(rng, 16, 2048)
     got = run(torch_dev, x, False, "split16")
     assert rel_l2_per_row(got, oracle.c2c(x, False)).max() <= 1e-6 * 11
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_split16_65536_matches_oracle(torch_dev, oracle, inverse):
+    """The precision workload's long case at its full size: N = 65536, batch 64, 10^k scaling."""
+    from oracle.oracle import rel_l2_per_row
+    n = 65536
+    rng = np.random.default_rng(8 + inverse)
+    x = scaled_input(rng, 64, n)
+    gold = oracle.c2c(x, inverse)
+    e16 = rel_l2_per_row(run(torch_dev, x, inverse, "split16"), gold)
+    e32 = rel_l2_per_row(run(torch_dev, x, inverse, "fp32"), gold)
+    tol = 1e-6 * math.log2(n)
+    print(f"\nN=65536 b64 inverse={inverse}: split16 max {e16.max():.3e} rms {np.sqrt((e16**2).mean()):.3e}; "
+          f"fp32 max {e32.max():.3e} rms {np.sqrt((e32**2).mean()):.3e}")
+    assert e16.max() <= tol
+    assert e32.max() <= tol
+
+
+@pytest.mark.parametrize("n,batch", [(32768, 3), (131072, 2), (65536, 1)])
+def test_split16_fourstep_lengths_and_in_place(torch_dev, oracle, n, batch):
+    from oracle.oracle import rel_l2_per_row
+    rng = np.random.default_rng(n + batch)
+    x = scaled_input(rng, batch, n)
+    gold = oracle.c2c(x, False)
+    got = run(torch_dev, x, False, "split16", in_place=True)
+    assert rel_l2_per_row(got, gold).max() <= 1e-6 * math.log2(n)
+
+
+def test_split16_65536_closed_forms(torch_dev):
+    """Impulse at n0 -> the phase ramp W^(n0 k); a zero row stays zero; a single tone lands in
+    its bin (checks the transposed store's index map)."""
+    n = 65536
+    x = np.zeros((3, n), np.complex64)
+    x[0, 5] = 1.0
+    t = np.arange(n)
+    x[2] = np.exp(2j * np.pi * 4097 * t / n).astype(np.complex64)
+    y = run(torch_dev, x, False, "split16")
+    ramp = np.exp(-2j * np.pi * 5 * t / n)
+    assert np.abs(y[0] - ramp).max() < 1e-5
+    assert np.all(y[1] == 0)
+    assert abs(y[2][4097] - n) < 1e-3 * n and np.abs(np.delete(y[2], 4097)).max() < 5e-2
