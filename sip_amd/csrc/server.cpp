@@ -475,12 +475,18 @@ void Server::scheduler_loop()
             const bool idle = inflight == 0 && batch.empty();
             if (inflight + add + q.bytes > budget && !idle) break;
             if (!cfg.batching && !batch.empty()) break;
+            // only small 1D requests share a batch (one ragged launch); any other request runs
+            // through its plain plan in a batch of its own, so large requests never queue behind
+            // one another on one slot's stream while other slots free up
+            const bool small = q.ragged && q.bytes <= kRaggedMaxRequestBytes;
+            if (!small && !batch.empty()) break;
             if (q.ragged && transforms + q.entry.count > kSlotTransforms) break;
             if (q.ragged) transforms += q.entry.count;
             batch.push_back(&q);
             tickets.push_back(pending.front());
             add += q.bytes;
             pending.pop_front();
+            if (!small) break;
         }
         if (batch.empty()) {
             // the head request waits for in-flight batches to release budget
