@@ -35,6 +35,19 @@ int32_t asdFftExtPlanSteps(void* handle, int32_t* steps, double* algorithmic_byt
 typedef struct aclTensor aclTensor;
 int32_t asdFftExtExecStep(void* handle, const aclTensor* input, const aclTensor* output, int32_t step);
 
+/* No reference counterpart.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): runs a C2C, R2C or
+ * C2R plan exactly as asdFftExecC2C / R2C / C2R do (same checks and status codes, the plan
+ * type chooses which), except that the plan's workspace is a device array inside the library
+ * (64 MiB, ASDFFT_NOALLOC_SCRATCH_BYTES, one per GPU, held until the process ends) instead of
+ * the buffer bound with asdFftSetWorkspace, so neither GetWorkspaceSize nor SetWorkspace is
+ * needed and the call allocates nothing.  A plan whose workspace exceeds the array is refused
+ * with ACL_ERROR_INVALID_PARAM.  Every call on one GPU uses that GPU's one array: calls that
+ * overlap on a GPU (other streams, other threads) race; serialise them.  Nothing in the array
+ * needs to start at zero (each call's first pass writes the parts its second pass reads). */
+#define ASDFFT_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int32_t asdFftExtExecNoAlloc(void* handle, const aclTensor* input, const aclTensor* output);
+
 /* ---- ragged batches -------------------------------------------------------------------
  * One exec transforms entries of different lengths, types and directions.  Entry e holds
  * `count` transforms stored back to back from in_base + in_offset (inputs) and

@@ -150,6 +150,7 @@ SIGNATURES = [
     ("asdFftExtDescribePlan", _I32, [_VP, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("asdFftExtPlanSteps", _I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     ("asdFftExtExecStep", _I32, [_VP, _VP, _VP, _I32]),
+    ("asdFftExtExecNoAlloc", _I32, [_VP, _VP, _VP]),
     ("asdFftExtMakePlanRagged", _I32, [_VP, ctypes.POINTER(RaggedEntry), _I64, ctypes.c_uint32]),
     ("asdFftExtExecRagged", _I32, [_VP, _VP, _VP]),
     ("asdFftExtRaggedInfo", _I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),

@@ -25,6 +25,7 @@
 #include "planner.h"
 #include "ragged.h"
 #include "shift.h"
+#include "static_scratch.h"
 #include "tensor_desc.h"
 
 namespace {
@@ -285,12 +286,25 @@ AspbStatus check_device(const asdfft::BuiltPlan& bp, const char* fn)
 }
 
 // asdFftExecV2 (fft_api.cpp:1446-1511): workspace guard, then the launch list
-// (only_step >= 0 launches that one step, for per-kernel timing).
-AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output, const char* fn, int only_step = -1)
+// (only_step >= 0 launches that one step, for per-kernel timing).  no_alloc: the plan's
+// workspace is the library's static device scratch (asdFftExtExecNoAlloc) instead of the
+// caller's bound buffer; plans needing more than its capacity are refused.
+AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output, const char* fn, int only_step = -1,
+                    bool no_alloc = false)
 {
     const asdfft::BuiltPlan& bp = p.built;
-    const AspbStatus ws = check_workspace(p, fn);
-    if (ws != ACL_SUCCESS) return ws;
+    void* workspace = p.workspace;
+    if (no_alloc) {
+        if (bp.workspace > asdfft::kStaticScratchBytes) {
+            log_error(fn, "the plan needs " + std::to_string(bp.workspace) + " bytes of workspace, more than the " +
+                              std::to_string(asdfft::kStaticScratchBytes) +
+                              "-byte static scratch; use asdFftExec* with a bound workspace");
+            return ACL_ERROR_INVALID_PARAM;
+        }
+    } else {
+        const AspbStatus ws = check_workspace(p, fn);
+        if (ws != ACL_SUCCESS) return ws;
+    }
     void* in = asdfft::data_address(input);
     void* out = asdfft::data_address(output);
     if (in == nullptr || out == nullptr) {
@@ -311,13 +325,21 @@ AspbStatus run_plan(FftPlan& p, const aclTensor* input, const aclTensor* output,
     }
     const AspbStatus dev = check_device(bp, fn);
     if (dev != ACL_SUCCESS) return dev;
-    char* bufs[3] = {static_cast<char*>(in), static_cast<char*>(out), static_cast<char*>(p.workspace)};
+    if (no_alloc && bp.workspace > 0) {
+        const hipError_t e = asdfft::static_scratch(&workspace);
+        if (e != hipSuccess) {
+            log_error(fn, std::string("static scratch lookup failed: ") + hipGetErrorString(e));
+            return ACL_ERROR_INTERNAL_ERROR;
+        }
+    }
+    char* bufs[3] = {static_cast<char*>(in), static_cast<char*>(out), static_cast<char*>(workspace)};
     return launch_steps(bp, bufs, nullptr, nullptr, static_cast<hipStream_t>(p.stream), fn, only_step);
 }
 
 template <typename MatchIn, typename MatchOut>
 AspbStatus exec_common(asdFftHandle handle, const aclTensor* input, const aclTensor* output, asdFftType want,
-                       MatchIn match_in, MatchOut match_out, const char* fn, int only_step = -1)
+                       MatchIn match_in, MatchOut match_out, const char* fn, int only_step = -1,
+                       bool no_alloc = false)
 {
     auto sp = lookup(handle);
     if (!sp) {
@@ -354,7 +376,7 @@ AspbStatus exec_common(asdFftHandle handle, const aclTensor* input, const aclTen
         log_error(fn, "step index out of range");
         return ACL_ERROR_INVALID_PARAM;
     }
-    return run_plan(p, input, output, fn, only_step);
+    return run_plan(p, input, output, fn, only_step, no_alloc);
 }
 
 // asdFftShift / asdIfftShift (core/fft/fftshift.cpp:101-138 and ifftshift.cpp): the
@@ -1062,6 +1084,28 @@ extern "C" int32_t asdFftExtExecStep(void* handle, const aclTensor* input, const
     if (t == AsdSip::ASCEND_FFT_C2R)
         return exec_common(handle, input, output, t, match_half_complex, match_real, "asdFftExtExecStep", step);
     return exec_common(handle, input, output, t, match_c2c, match_c2c, "asdFftExtExecStep", step);
+}
+
+static_assert(ASDFFT_NOALLOC_SCRATCH_BYTES == asdfft::kStaticScratchBytes, "documented scratch size");
+
+extern "C" int32_t asdFftExtExecNoAlloc(void* handle, const aclTensor* input, const aclTensor* output)
+{
+    const char* fn = "asdFftExtExecNoAlloc";
+    auto sp = lookup(handle);
+    if (!sp) {
+        log_error(fn, "invalid handle");
+        return AsdSip::ACL_ERROR_INVALID_PARAM;
+    }
+    asdFftType t;
+    {
+        std::lock_guard<std::mutex> lock(sp->mu);
+        t = sp->type;
+    }
+    if (t == AsdSip::ASCEND_FFT_R2C) return exec_common(handle, input, output, t, match_real, match_half_complex, fn, -1, true);
+    if (t == AsdSip::ASCEND_FFT_C2R) return exec_common(handle, input, output, t, match_half_complex, match_real, fn, -1, true);
+    if (t == AsdSip::ASCEND_FFT_C2C) return exec_common(handle, input, output, t, match_c2c, match_c2c, fn, -1, true);
+    log_error(fn, "only C2C, R2C and C2R plans run through this entry point");
+    return AsdSip::ACL_ERROR_INVALID_PARAM;
 }
 
 extern "C" int32_t asdFftExtMakePlanRagged(void* handle, const asdFftExtRaggedEntry* entries, int64_t num_entries,
