@@ -422,6 +422,11 @@ def bench_ragged(torch, dev, stream, reps, warmup):
         p_stream = RaggedPlan(grouped)
         p_graph = RaggedPlan(grouped, graph=True)
         p_scat = RaggedPlan(scattered, graph=True)
+        # one concurrent launch per register budget (EMAX 16 / 32 / 48) instead of the default
+        # single launch of every class
+        p_budget = RaggedPlan(grouped, per_budget=True)
+        res["plan_describe_head"] = p_stream.describe().splitlines()[:2]
+        res["per_budget_describe_head"] = [l for l in p_budget.describe().splitlines() if "launch" in l]
         t_tune = time.perf_counter()
         p_tuned = RaggedPlan(grouped, graph=True, autotune=True)
         res["autotune_plan_s"] = round(time.perf_counter() - t_tune, 2)
@@ -456,6 +461,7 @@ def bench_ragged(torch, dev, stream, reps, warmup):
             "ragged_graph": lambda: p_graph.execute(xin, yout, stream.cuda_stream),
             "ragged_graph_scattered": lambda: p_scat.execute(xs, ys, stream.cuda_stream),
             "ragged_graph_autotuned": lambda: p_tuned.execute(xin, yout, stream.cuda_stream),
+            "ragged_per_budget_launches": lambda: p_budget.execute(xin, yout, stream.cuda_stream),
             "plain_per_size_loop": run_plain,
         }
         # the per-size loop captured once in a CUDA graph: the plain kernels without host overhead
@@ -1116,9 +1122,12 @@ def main():
                     st = L.asdFftExtExecStep(j["plan"].handle, ti.ptr, to.ptr, si)
                     if st != 0:
                         raise RuntimeError(f"{j['name']} step {si}: {st}")
+                # the launch's average duration: HIP events around `samples` back-to-back launches
+                # on its stream (the profiler's kernel averages compare with this); the
+                # per-sample median adds each sample's event overhead (about 2-3 us)
+                kms = event_time_ms(torch, stream, one, a.samples)
                 kt = device_times_us(torch, stream, one, a.samples)
-                kms = kt["mean"] * 1e-3
-                launches.append({"workload": j["name"], "step": si, "us": kms * 1e3, "us_median": kt["median"],
+                launches.append({"workload": j["name"], "step": si, "us": kms * 1e3, "us_sample_median": kt["median"],
                                  "bytes": sb, "GBps": sb / kms / 1e6, "ring_pairs": len(ring)})
             for ti, to in ring[1:]:
                 ti.close()

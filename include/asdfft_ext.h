@@ -67,6 +67,9 @@ typedef struct asdFftExtRaggedEntry {
 
 #define ASDFFT_RAGGED_GRAPH 0x1u    /* exec replays an instantiated hipGraph of the launches */
 #define ASDFFT_RAGGED_AUTOTUNE 0x2u /* time candidate schedules on the device while planning */
+#define ASDFFT_RAGGED_PER_BUDGET 0x4u /* one concurrent launch per register budget (EMAX 16 / 32 /
+                                       48) instead of the default single launch of every class at
+                                       the largest budget (measured slower; kept for comparison) */
 
 /* No reference counterpart (batching form).  Plans a ragged batch on a handle from
  * asdFftCreate.  Outputs of different entries must not overlap.  Returns

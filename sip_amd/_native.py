@@ -104,6 +104,7 @@ ASDFFT_SHIFT_FFT = 1
 ASDFFT_SHIFT_IFFT = 2
 ASDFFT_RAGGED_GRAPH = 0x1
 ASDFFT_RAGGED_AUTOTUNE = 0x2
+ASDFFT_RAGGED_PER_BUDGET = 0x4
 
 
 class RaggedEntry(ctypes.Structure):

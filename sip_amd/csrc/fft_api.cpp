@@ -1127,7 +1127,7 @@ extern "C" int32_t asdFftExtMakePlanRagged(void* handle, const asdFftExtRaggedEn
         log_error(fn, "entries must be a non-empty array");
         return ACL_ERROR_INVALID_PARAM;
     }
-    if ((flags & ~(ASDFFT_RAGGED_GRAPH | ASDFFT_RAGGED_AUTOTUNE)) != 0) {
+    if ((flags & ~(ASDFFT_RAGGED_GRAPH | ASDFFT_RAGGED_AUTOTUNE | ASDFFT_RAGGED_PER_BUDGET)) != 0) {
         log_error(fn, "unknown flags");
         return ACL_ERROR_INVALID_PARAM;
     }
@@ -1164,6 +1164,7 @@ extern "C" int32_t asdFftExtMakePlanRagged(void* handle, const asdFftExtRaggedEn
     asdfft::RaggedOptions opt;
     opt.graph = (flags & ASDFFT_RAGGED_GRAPH) != 0;
     opt.autotune = (flags & ASDFFT_RAGGED_AUTOTUNE) != 0;
+    opt.one_launch = (flags & ASDFFT_RAGGED_PER_BUDGET) == 0;
     auto plan = std::make_unique<asdfft::RaggedPlan>();
     bool unsupported = false;
     std::string why;

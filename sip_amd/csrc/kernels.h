@@ -36,6 +36,7 @@ enum Variant : int {
     kVarColsShiftOutSplit = 19, // kVarColsShiftOut exchanging through one LDS plane (specialised only)
     kVarRowsC2CSplit = 20,      // rows C2C exchanging through one LDS plane (specialised only)
     kVarRowsPair2 = 21,         // 2D rows in pairs n, n + X/2 with the columns' first radix-2 stage (specialised only)
+    kVarRowsC2CTwPre = 22,      // rows C2C with every pass's twiddles loaded at kernel start (specialised only)
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a

@@ -7,7 +7,9 @@
 #include <algorithm>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <sstream>
+#include <unordered_map>
 
 namespace asdfft {
 
@@ -173,6 +175,26 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
         found = true;
         if (want_tpb <= 0) break;  // first feasible register budget (EMAX 16 preferred)
     }
+    // Launches of one transform per workgroup and at most two per CU: where a specialised
+    // instance exists, the EMAX-16 geometry with more threads per transform (e.g. 6000 on
+    // 600 threads x 10 points instead of 200 x 30), so each CU has more waves and shorter
+    // per-thread chains to hide the latency of a single transform.  ASDFFT_WIDE_ROWS=0 turns
+    // it off.
+    const char* wr = getenv("ASDFFT_WIDE_ROWS");
+    const char* nst = getenv("ASDFFT_NO_STATIC");  // the wide geometry exists only as an instance
+    if (found && variant == kVarRowsC2C && want_tpb <= 0 && best.tpb == 1 && best.emax > 16 &&
+        count <= 2 * kNumCUs && !(wr != nullptr && wr[0] == '0') && !(nst != nullptr && nst[0] == '1')) {
+        int64_t t = 0;
+        for (int r : radices) {
+            t = (r > 16) ? 0 : std::max<int64_t>(t, (n / r + 16 / r - 1) / (16 / r));
+            if (t == 0) break;
+        }
+        if (t > 0 && t <= 1024 &&
+            find_static(kVarRowsC2C, 16, (int)n, (int)t, 0, radices.data(), (int)radices.size()) >= 0) {
+            best.emax = 16;
+            best.threads = (int)t;
+        }
+    }
     // ASDFFT_ROWS_SPLIT=1: rows C2C on a specialised one-plane instance for the chosen geometry
     // (half the LDS, twice the resident workgroups).  Opt-in: measured no faster than the
     // two-plane kernel (1024 x 8192: 28.6 vs 28.7 us; 1000 / 2187 / 6000 x 256: 9-10% slower),
@@ -319,6 +341,16 @@ int xcd_remap_for(bool cols_mapping, int tpb)
     return (cols_mapping && tpb < 16) ? 1 : 0;
 }
 
+// Row launches of at most this many workgroups take a twiddle-preloading instance where one
+// exists (latency-bound launches: a few workgroups per CU, so the extra registers cost no
+// occupancy that matters); ASDFFT_TW_PRELOAD=0 / 1 turns them off / on for every row launch.
+bool tw_preload_wanted(int64_t grid)
+{
+    const char* e = getenv("ASDFFT_TW_PRELOAD");
+    if (e != nullptr && (e[0] == '0' || e[0] == '1')) return e[0] == '1';
+    return grid <= 1024;
+}
+
 // A Stockham launch.  Addressing: transform t at (t % cols)*inner + (t / cols)*outer.
 struct StepIo {
     int64_t count = 0;
@@ -388,6 +420,15 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     if (ns == nullptr || ns[0] != '1') {
         st->shape.static_id = find_static(st->shape.variant, c.emax, (int)n, c.threads, c.tpb, c.radices.data(),
                                           (int)c.radices.size());
+        // small row launches: the instance with every pass's twiddles loaded at kernel start
+        if (st->shape.variant == kVarRowsC2C && tw_preload_wanted(st->shape.grid)) {
+            const int pre = find_static(kVarRowsC2CTwPre, c.emax, (int)n, c.threads, c.tpb, c.radices.data(),
+                                        (int)c.radices.size());
+            if (pre >= 0) {
+                st->shape.variant = kVarRowsC2CTwPre;
+                st->shape.static_id = pre;
+            }
+        }
     }
     std::ostringstream os;
     os << "stockham variant=" << st->shape.variant << " n=" << n << " count=" << io.count << " radices=";
@@ -914,7 +955,7 @@ bool pair2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
 
 }  // namespace
 
-std::vector<int> choose_radices(int64_t n)
+static std::vector<int> search_radices(int64_t n)
 {
     if (n <= 1) return {};
     std::vector<int> e(kNumPrimes, 0);
@@ -932,6 +973,23 @@ std::vector<int> choose_radices(int64_t n)
     std::vector<int> r = b.radices;
     std::sort(r.begin(), r.end(), [](int x, int y) { return x > y; });
     if ((int)r.size() > kMaxPasses) return {};
+    return r;
+}
+
+// The search above, once per length for the process (the serving loop asks at every
+// submission; the search itself takes microseconds to tens of microseconds).
+std::vector<int> choose_radices(int64_t n)
+{
+    static std::mutex mu;
+    static auto* cache = new std::unordered_map<int64_t, std::vector<int>>();
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache->find(n);
+        if (it != cache->end()) return it->second;
+    }
+    std::vector<int> r = search_radices(n);
+    std::lock_guard<std::mutex> lock(mu);
+    if (cache->size() < 65536) (*cache)[n] = r;
     return r;
 }
 

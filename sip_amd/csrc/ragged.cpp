@@ -355,6 +355,14 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
         if (!why.empty()) return why;
     }
 
+    // one launch for the whole batch: every class takes the largest register budget (a
+    // class keeps its thread count; the extra butterfly slots of a larger budget stay idle)
+    if (opt.one_launch) {
+        int top = 0;
+        for (const RaggedClassInfo& c : plan->classes) top = std::max(top, c.emax);
+        for (RaggedClassInfo& c : plan->classes) c.emax = top;
+    }
+
     // ---- launches: one per register budget
     for (int emax : {16, 32, 48}) {
         RaggedLaunchPlan L;

@@ -53,6 +53,8 @@ struct ClassSchedule {
 struct RaggedOptions {
     bool graph = false;     // replay the launches from an instantiated hipGraph
     bool autotune = false;  // time candidate schedules on the device at plan time
+    bool one_launch = true;  // every class in one launch at the largest register budget (false:
+                             // one concurrent launch per register budget)
     int block16 = 0;        // workgroup size of the EMAX-16 launch (0: automatic)
     int block32 = 0;        // workgroup size of the EMAX-32 launch (0: automatic)
     int64_t max_grid = 0;   // workgroups per launch at most (0: one per tile)

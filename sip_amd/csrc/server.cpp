@@ -135,6 +135,7 @@ public:
     RaggedClass* d_pool = nullptr;
     std::vector<PoolClass> pool;
     std::map<std::string, int> pool_index;
+    std::unordered_map<uint64_t, int> pool_fast;  // (kind, direction, n) -> pool index, no key string
     std::vector<TablePtr> tables;
     std::mutex plan_mu;  // plain plans are created by submitters and read by the scheduler
     std::map<std::string, std::unique_ptr<PlainPlan>> plans;
@@ -236,7 +237,10 @@ Server::~Server()
 // Index of the resident class for this entry (created and uploaded on first use), or -1.
 int Server::pool_class(const RaggedEntry& e)
 {
+    const uint64_t fast = ((uint64_t)e.n << 3) | ((uint64_t)e.inverse << 2) | (uint64_t)e.kind;
     std::lock_guard<std::mutex> lock(pool_mu);
+    auto f = pool_fast.find(fast);
+    if (f != pool_fast.end()) return f->second;
     const std::string key = ragged_class_key(e);
     auto it = pool_index.find(key);
     if (it != pool_index.end()) return it->second;
@@ -267,6 +271,7 @@ int Server::pool_class(const RaggedEntry& e)
     }
     pool.push_back(pc);
     pool_index[key] = idx;
+    pool_fast[fast] = idx;
     return idx;
 }
 
@@ -370,6 +375,8 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
     const int64_t* d_offsets = reinterpret_cast<const int64_t*>(s.d_stage);
     const RaggedTile* d_tiles = reinterpret_cast<const RaggedTile*>(s.d_stage + tiles_at);
     if (!tiles.empty()) {
+        // one copy of the whole work table from pinned memory (a kernel reading the table
+        // from host memory instead measured slower: 50k req/s p50 2.7 ms against 0.14 ms)
         if (hipMemcpyAsync(s.d_stage, s.h_stage, tiles_at + sizeof(RaggedTile) * tiles.size(), hipMemcpyHostToDevice,
                            s.stream) != hipSuccess) {
             return 3;

@@ -389,10 +389,12 @@ __device__ __forceinline__ void lds_barrier()
 // arithmetic out of that loop and keeping all of it live at once (hundreds of VGPRs).
 // LB: exchange barriers wait for LDS traffic only (lds_barrier), so a persistent
 // workgroup's stores of the previous tile keep draining while the next tile runs.
-template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false>
+// PRE: this pass's twiddles were loaded into registers at kernel start (twr[b * (R - 1) + r - 1]
+// for butterfly b of the thread), so no table load waits inside the pass.
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false>
 __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
                                               bool first, bool last, float* lre, float* lim, int tt, int tid,
-                                              bool live, const Io& io)
+                                              bool live, const Io& io, const float2* twr = nullptr)
 {
     float2 v[EMAX];  // pass-local: nothing is carried between passes except through LDS
     constexpr int B = EMAX / R;
@@ -456,10 +458,15 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
         const int j = tid + b * T;
         if (j < nr) {
             if (span > 1) {
-                const int k = kmod(j);
-                const float2* w = twp + k * (R - 1);
+                if constexpr (PRE) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
+                    for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], twr[b * (R - 1) + r - 1]);
+                } else {
+                    const int k = kmod(j);
+                    const float2* w = twp + k * (R - 1);
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
+                }
             }
             Dft<R>::run(&v[b * R]);
         }
@@ -773,11 +780,14 @@ __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, fl
 }
 
 // Column kernels know their workgroup (T x TPB) at compile time and take it as the launch
-// bound, so an EMAX-16 column tile may use 512 threads; row kernels keep tpb at run time.
+// bound, so an EMAX-16 column tile may use 512 threads; row kernels keep tpb at run time, and
+// a row instance wider than the register budget's block (one transform per workgroup) takes
+// its own thread count, rounded up to whole wavefronts.
 template <int EMAX, int MAP, int T, int TPB>
 constexpr int static_block_bound()
 {
-    return MAP == kMapCols ? T * TPB : max_block<EMAX>();
+    if constexpr (MAP == kMapCols) return T * TPB;
+    return T > max_block<EMAX>() ? (T + 63) / 64 * 64 : max_block<EMAX>();
 }
 
 template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int... Rs>
@@ -796,6 +806,76 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
     Io io;
     thread_setup<MAP, PRO, EPI>(a, g, tt, tid, blk, live, io);
     static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, lre, lim, tt, tid, live, io);
+    lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
+}
+
+// ------------------------------------------------------------------ twiddles preloaded
+// Specialised row kernels for small launches (a few hundred workgroups, latency-bound): every
+// pass's twiddles are loaded into registers at kernel start, beside the input loads, instead
+// of one dependent table load per pass after each LDS exchange.  Pass p of radix R takes
+// (EMAX / R) * (R - 1) entries (none for the first pass, whose span is 1).
+template <int EMAX, int... Rs>
+constexpr int tw_preload_count()
+{
+    int span = 1, total = 0;
+    for (int r : {Rs...}) {
+        if (span > 1) total += (EMAX / r) * (r - 1);
+        span *= r;
+    }
+    return total;
+}
+
+template <int EMAX, int N, int T, int SPAN, int TWOFF, int R, int... Rest>
+__device__ __forceinline__ void preload_twiddles(const FftArgs& a, int tid, float2* twr)
+{
+    constexpr int B = EMAX / R;
+    constexpr int NR = N / R;
+    if constexpr (SPAN > 1) {
+        const float2* twp = reinterpret_cast<const float2*>(a.tw) + TWOFF;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = tid + b * T;
+            const int k = (j < NR) ? j % SPAN : 0;  // threads past the pass's range read entry 0
+#pragma unroll
+            for (int r = 1; r < R; ++r) twr[b * (R - 1) + r - 1] = twp[k * (R - 1) + r - 1];
+        }
+    }
+    if constexpr (sizeof...(Rest) > 0) {
+        constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
+        preload_twiddles<EMAX, N, T, SPAN * R, kNextOff, Rest...>(a, tid, twr + (SPAN > 1 ? B * (R - 1) : 0));
+    }
+}
+
+template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
+__device__ __forceinline__ void static_passes_pre(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt,
+                                                  int tid, bool live, const Io& io, const float2* twr)
+{
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, true>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0,
+                                                             lre, lim, tt, tid, live, io, twr);
+    if constexpr (sizeof...(Rest) > 0) {
+        constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
+        static_passes_pre<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(
+            a, g, lre, lim, tt, tid, live, io, twr + (SPAN > 1 ? (EMAX / R) * (R - 1) : 0));
+    }
+}
+
+template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int... Rs>
+__global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) stockham_static_pre_kernel(const FftArgs a)
+{
+    extern __shared__ float smem[];
+    constexpr int kTstride = N + (N >> 4) + 1;
+    float* lre = smem;
+    float* lim = smem + a.lds_plane;
+    const Geo g{N, T, (MAP == kMapRows) ? a.tpb : TPB, kTstride};
+    int tt, tid;
+    int64_t blk;
+    bool live;
+    Io io;
+    thread_setup<MAP, PRO, EPI>(a, g, tt, tid, blk, live, io);
+    constexpr int kPre = tw_preload_count<EMAX, Rs...>();
+    float2 twr[kPre > 0 ? kPre : 1];
+    preload_twiddles<EMAX, N, T, 1, 0, Rs...>(a, tid, twr);
+    static_passes_pre<EMAX, MAP, PRO, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, lre, lim, tt, tid, live, io, twr);
     lds_epilogue<MAP, EPI>(a, g, lre, lim, tt, tid, blk, live, io);
 }
 
@@ -954,6 +1034,7 @@ template <> struct VariantTraits<kVarRowsShiftIn> { static constexpr int map = k
 template <> struct VariantTraits<kVarRowsShiftBoth> { static constexpr int map = kMapRows, pro = kProShift, epi = kEpiShift; };
 template <> struct VariantTraits<kVarColsShiftOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiShift; };
 template <> struct VariantTraits<kVarRowsPair2> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiPairRadix2; };
+template <> struct VariantTraits<kVarRowsC2CTwPre> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiComplex; };
 
 // One instantiation per kernel, so the LDS limit is raised once per kernel and device.
 template <auto Kern>
@@ -984,6 +1065,9 @@ hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t 
         return launch_kernel<stockham_split_kernel<EMAX, kMapCols, kEpiShift, N, T, TPB, Rs...>>(s, args, stream);
     } else if constexpr (V == kVarRowsC2CSplit) {
         return launch_kernel<stockham_split_kernel<EMAX, kMapRows, kEpiComplex, N, T, TPB, Rs...>>(s, args, stream);
+    } else if constexpr (V == kVarRowsC2CTwPre) {
+        return launch_kernel<stockham_static_pre_kernel<EMAX, kMapRows, kProComplex, kEpiComplex, N, T, TPB, Rs...>>(
+            s, args, stream);
     } else {
         using Tr = VariantTraits<V>;
         return launch_kernel<stockham_static_kernel<EMAX, Tr::map, Tr::pro, Tr::epi, N, T, TPB, Rs...>>(s, args,

@@ -253,7 +253,7 @@ class RaggedPlan:
     order, each starting on an 8-byte boundary.
     """
 
-    def __init__(self, entries, graph: bool = False, autotune: bool = False):
+    def __init__(self, entries, graph: bool = False, autotune: bool = False, per_budget: bool = False):
         self.L = N.lib()
         rows = []
         ioff = ooff = 0
@@ -279,7 +279,8 @@ class RaggedPlan:
         h = ctypes.c_void_p()
         N.check("asdFftCreate", self.L.asdFftCreate(ctypes.byref(h)))
         self.handle = h
-        flags = (N.ASDFFT_RAGGED_GRAPH if graph else 0) | (N.ASDFFT_RAGGED_AUTOTUNE if autotune else 0)
+        flags = (N.ASDFFT_RAGGED_GRAPH if graph else 0) | (N.ASDFFT_RAGGED_AUTOTUNE if autotune else 0) | \
+            (N.ASDFFT_RAGGED_PER_BUDGET if per_budget else 0)
         try:
             N.check("asdFftExtMakePlanRagged", self.L.asdFftExtMakePlanRagged(h, arr, len(rows), flags))
         except Exception:

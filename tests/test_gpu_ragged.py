@@ -246,3 +246,23 @@ def test_autotuned_plan_matches_oracle(torch_dev, oracle):
         check_entries(oracle, plan.entries, inputs, outs)
     finally:
         plan.close()
+
+
+def test_launch_forms_match_oracle(torch_dev, oracle):
+    """The default single launch (every class at the largest register budget: classes
+    scheduled for a smaller budget run with idle butterfly slots) and the per-budget form
+    (ASDFFT_RAGGED_PER_BUDGET, one concurrent launch per budget) both meet the bound, C2C / R2C /
+    C2R, both directions, lengths from every budget."""
+    from sip_amd.fft import RaggedPlan
+    rng = np.random.default_rng(18)
+    spec = [("c2c", 64, 40, False), ("c2c", 4096, 3, True), ("r2c", 8192, 4, False), ("c2r", 1000, 5, True),
+            ("c2c", 16384, 2, False), ("r2c", 243, 9, True), ("c2c", 14400, 2, True), ("c2r", 6000, 3, False)]
+    inputs = [make_input(rng, k, n, c) for k, n, c, _ in spec]
+    for per_budget, launches in ((False, "1 launch(es)"), (True, "3 launch(es)")):
+        plan = RaggedPlan(spec, per_budget=per_budget)
+        try:
+            assert launches in plan.describe(), plan.describe()
+            outs = run_ragged(torch_dev, plan, inputs)
+            check_entries(oracle, plan.entries, inputs, outs)
+        finally:
+            plan.close()

@@ -84,9 +84,19 @@ def test_valid_plan_and_info(L):
         assert steps.value == 1  # one register budget -> one launch
 
 
-def test_two_register_budgets_two_launches(L):
+def test_two_register_budgets_one_launch_or_two(L):
+    """Classes of two register budgets share one launch by default; with
+    ASDFFT_RAGGED_PER_BUDGET each budget gets a launch of its own."""
+    rows = [(C2C, FWD, 4096, 1, 0, 0), (C2C, FWD, 16384, 1, 32768, 32768)]
     with Handle(L) as h:
-        assert make(L, h, [(C2C, FWD, 4096, 1, 0, 0), (C2C, FWD, 16384, 1, 32768, 32768)]) == OK
+        assert make(L, h, rows) == OK
+        steps = ctypes.c_int32()
+        arr = (ctypes.c_double * 2)()
+        assert L.asdFftExtPlanSteps(h, ctypes.byref(steps), arr) == OK
+        assert steps.value == 1
+        assert arr[0] == 16.0 * 4096 + 16.0 * 16384
+    with Handle(L) as h:
+        assert L.asdFftExtMakePlanRagged(h, entries(*rows), len(rows), N.ASDFFT_RAGGED_PER_BUDGET) == OK
         steps = ctypes.c_int32()
         arr = (ctypes.c_double * 2)()
         assert L.asdFftExtPlanSteps(h, ctypes.byref(steps), arr) == OK
@@ -134,7 +144,7 @@ def test_plan_rules(L):
     with Handle(L) as h:
         assert L.asdFftExtMakePlanRagged(h, None, 1, 0) == INVALID
         assert L.asdFftExtMakePlanRagged(h, arr, 0, 0) == INVALID
-        assert L.asdFftExtMakePlanRagged(h, arr, len(GOOD), 0x4) == INVALID  # unknown flag
+        assert L.asdFftExtMakePlanRagged(h, arr, len(GOOD), 0x8) == INVALID  # unknown flag
         assert L.asdFftExtMakePlanRagged(h, arr, len(GOOD), N.ASDFFT_RAGGED_GRAPH) == OK
         assert L.asdFftExtMakePlanRagged(h, arr, len(GOOD), 0) == INVALID  # already initialised
     with Handle(L) as h:
