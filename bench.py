@@ -425,7 +425,15 @@ def bench_ragged(torch, dev, stream, reps, warmup):
         # one concurrent launch per register budget (EMAX 16 / 32 / 48) instead of the default
         # single launch of every class
         p_budget = RaggedPlan(grouped, per_budget=True)
-        res["plan_describe_head"] = p_stream.describe().splitlines()[:2]
+        # the EMAX-16 classes on wide workgroups (up to 1024 threads per transform) in a launch of
+        # their own beside the single launch of the others
+        os.environ["ASDFFT_RAGGED_WIDE16"] = "1"
+        try:
+            p_wide = RaggedPlan(grouped)
+        finally:
+            os.environ.pop("ASDFFT_RAGGED_WIDE16", None)
+        res["wide16_describe_head"] = [l for l in p_wide.describe().splitlines() if "launch" in l]
+        res["plan_describe_head"] = [l for l in p_stream.describe().splitlines() if "launch" in l]
         res["per_budget_describe_head"] = [l for l in p_budget.describe().splitlines() if "launch" in l]
         t_tune = time.perf_counter()
         p_tuned = RaggedPlan(grouped, graph=True, autotune=True)
@@ -462,6 +470,7 @@ def bench_ragged(torch, dev, stream, reps, warmup):
             "ragged_graph_scattered": lambda: p_scat.execute(xs, ys, stream.cuda_stream),
             "ragged_graph_autotuned": lambda: p_tuned.execute(xin, yout, stream.cuda_stream),
             "ragged_per_budget_launches": lambda: p_budget.execute(xin, yout, stream.cuda_stream),
+            "ragged_wide16_two_launches": lambda: p_wide.execute(xin, yout, stream.cuda_stream),
             "plain_per_size_loop": run_plain,
         }
         # the per-size loop captured once in a CUDA graph: the plain kernels without host overhead

@@ -173,7 +173,7 @@ std::string ragged_class_key(const RaggedEntry& e)
     return os.str();
 }
 
-std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, bool* unsupported)
+std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, bool* unsupported, int max_t16)
 {
     RaggedClassInfo& c = *cp;
     ClassSchedule sch;
@@ -187,10 +187,11 @@ std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, boo
     if (prod != c.n || sch.radices.empty() || (int)sch.radices.size() > kMaxPasses) {
         return "schedule for " + c.key + " does not factor its length";
     }
+    auto limit = [&](int emax) { return emax <= 16 ? std::max(max_t16, ragged_block_for(16)) : ragged_block_for(emax); };
     auto fit = [&](const std::vector<int>& r, int* emax_out, int* t_out) {
         for (int emax : {16, 32, 48}) {
             const int t = min_threads(c.n, r, emax);
-            if (t > 0 && t <= ragged_block_for(emax)) {
+            if (t > 0 && t <= limit(emax)) {
                 *emax_out = emax;
                 *t_out = t;
                 return true;
@@ -209,11 +210,24 @@ std::string ragged_schedule(RaggedClassInfo* cp, const ClassSchedule* fixed, boo
             }
         }
     }
+    if (fixed == nullptr && max_t16 > ragged_block_for(16) && emax > 16) {
+        // wide EMAX-16 workgroups: a multiset with up to three more passes that fits 16 points
+        // per thread in at most max_t16 threads beats a larger register budget
+        for (const std::vector<int>& alt : radix_multisets(c.n, 3, 64)) {
+            const int t16 = min_threads(c.n, alt, 16);
+            if (t16 > 0 && t16 <= max_t16) {
+                sch.radices = alt;
+                emax = 16;
+                t = t16;
+                break;
+            }
+        }
+    }
     if (emax > 0) {
         c.emax = emax;
         sch.threads = std::max(sch.threads, t);
     }
-    if (c.emax == 0 || sch.threads > kRaggedMaxBlock) {
+    if (c.emax == 0 || sch.threads > ragged_max_block(c.emax)) {
         *unsupported = true;
         return "no register budget holds " + c.key;
     }
@@ -351,16 +365,29 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
     for (size_t ci = 0; ci < plan->classes.size(); ci++) {
         RaggedClassInfo& c = plan->classes[ci];
         auto f = opt.fixed.find(c.key);
-        const std::string why = ragged_schedule(&c, f != opt.fixed.end() ? &f->second : nullptr, unsupported);
+        const std::string why = ragged_schedule(&c, f != opt.fixed.end() ? &f->second : nullptr, unsupported,
+                                                opt.wide16 ? kRaggedMaxBlock16 : ragged_block_for(16));
         if (!why.empty()) return why;
     }
 
     // one launch for the whole batch: every class takes the largest register budget (a
-    // class keeps its thread count; the extra butterfly slots of a larger budget stay idle)
+    // class keeps its thread count; the extra butterfly slots of a larger budget stay idle).
+    // With wide EMAX-16 schedules (up to 1024 threads per transform, more than the larger
+    // budgets' 512-thread workgroups hold) the EMAX-16 classes keep a launch of their own and
+    // only the others share the largest budget: two launches at most.
     if (opt.one_launch) {
-        int top = 0;
-        for (const RaggedClassInfo& c : plan->classes) top = std::max(top, c.emax);
-        for (RaggedClassInfo& c : plan->classes) c.emax = top;
+        int top = 0, top_rest = 0;
+        for (const RaggedClassInfo& c : plan->classes) {
+            top = std::max(top, c.emax);
+            if (c.emax > 16) top_rest = std::max(top_rest, c.emax);
+        }
+        for (RaggedClassInfo& c : plan->classes) {
+            if (!opt.wide16) {
+                c.emax = top;
+            } else if (c.emax > 16) {
+                c.emax = top_rest;
+            }
+        }
     }
 
     // ---- launches: one per register budget
@@ -379,12 +406,17 @@ std::string build_once(const std::vector<RaggedEntry>& entries, const RaggedOpti
         const int want = (emax == 16) ? opt.block16 : (emax == 32 ? opt.block32 : 0);
         int block = want > 0 ? want : ragged_block_for(emax);
         block = std::max(block, (maxT + 63) / 64 * 64);
-        block = std::min(block, kRaggedMaxBlock);
+        block = std::min(block, ragged_max_block(emax));
         L.block = block;
         // whole workgroups of transforms: tiles fill every lane that LDS allows
         const size_t budget = std::min(kLdsLimit, std::max(kLdsFloor, one * (size_t)std::max(1, block / maxT)));
         for (int ci : L.class_ids) {
             RaggedClassInfo& c = plan->classes[ci];
+            if (c.sched.threads > block) {  // every transform of a tile needs its threads in the workgroup
+                *unsupported = true;
+                return "schedule of " + c.key + " needs " + std::to_string(c.sched.threads) +
+                       " threads, more than its launch's workgroup of " + std::to_string(block);
+            }
             int tpb = block / c.sched.threads;
             if (c.sched.tpb > 0) tpb = std::min(tpb, c.sched.tpb);
             tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, c.transforms));

@@ -18,8 +18,11 @@
 
 namespace asdfft {
 
-// Largest workgroup of the ragged kernel (its __launch_bounds__).
+// Largest workgroup of the ragged kernel (its __launch_bounds__): 1024 threads for the EMAX-16
+// kernel (at most 128 VGPRs, which that budget meets), 512 for the EMAX-32 and EMAX-48 ones.
 constexpr int kRaggedMaxBlock = 512;
+constexpr int kRaggedMaxBlock16 = 1024;
+constexpr int ragged_max_block(int emax) { return emax <= 16 ? kRaggedMaxBlock16 : kRaggedMaxBlock; }
 // Dynamic LDS a ragged launch may take: the workgroup's 160 KiB less room for the
 // kernel's static tile index.
 constexpr size_t kRaggedLdsLimit = 160 * 1024 - 256;
@@ -55,6 +58,8 @@ struct RaggedOptions {
     bool autotune = false;  // time candidate schedules on the device at plan time
     bool one_launch = true;  // every class in one launch at the largest register budget (false:
                              // one concurrent launch per register budget)
+    bool wide16 = false;     // EMAX-16 schedules of up to 1024 threads per transform before larger budgets
+                             // (the EMAX-16 classes then take a launch of their own: two launches)
     int block16 = 0;        // workgroup size of the EMAX-16 launch (0: automatic)
     int block32 = 0;        // workgroup size of the EMAX-32 launch (0: automatic)
     int64_t max_grid = 0;   // workgroups per launch at most (0: one per tile)
@@ -138,7 +143,9 @@ std::string ragged_class_key(const RaggedEntry& e);  // "c2c:f:4096", ...
 int64_t ragged_in_bytes(const RaggedEntry& e);       // bytes of one transform's input
 int64_t ragged_out_bytes(const RaggedEntry& e);
 // Fills c->sched (the given schedule, or the planner's default) and c->emax; "" on success.
-std::string ragged_schedule(RaggedClassInfo* c, const ClassSchedule* fixed, bool* unsupported);
+// max_t16: most threads per transform at the EMAX-16 budget (ragged_block_for(16) unless the
+// caller runs wide EMAX-16 workgroups).
+std::string ragged_schedule(RaggedClassInfo* c, const ClassSchedule* fixed, bool* unsupported, int max_t16 = 256);
 size_t ragged_class_lds(const RaggedClassInfo& c, int tpb);
 // Device descriptor of a class at tpb transforms per tile; uploads its twiddle tables.
 bool ragged_device_class(const RaggedClassInfo& c, const ClassShape& s, int tpb, RaggedClass* rc,

@@ -78,7 +78,7 @@ __device__ __forceinline__ void ragged_tile(const RaggedArgs& r, const FftArgs& 
 }
 
 template <int EMAX>
-__global__ void __launch_bounds__(kRaggedMaxBlock) __attribute__((amdgpu_waves_per_eu(min_waves<EMAX>())))
+__global__ void __launch_bounds__(ragged_max_block(EMAX)) __attribute__((amdgpu_waves_per_eu(min_waves<EMAX>())))
 ragged_kernel(const RaggedArgs r, const RaggedClass* __restrict__ classes, const RaggedTile* __restrict__ tiles)
 {
     extern __shared__ float smem[];
@@ -158,7 +158,8 @@ hipError_t ASDFFT_RNAME(launch_ragged, ASDFFT_RAGGED_EMAX)(int64_t grid, int blo
                                                             const RaggedArgs& args, const RaggedClass* classes,
                                                             const RaggedTile* tiles, hipStream_t stream)
 {
-    if (block <= 0 || block > kRaggedMaxBlock || block % 64 != 0 || grid <= 0) return hipErrorInvalidValue;
+    if (block <= 0 || block > ragged_max_block(ASDFFT_RAGGED_EMAX) || block % 64 != 0 || grid <= 0)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(ragged_kernel<ASDFFT_RAGGED_EMAX>, dim3((unsigned)grid), dim3((unsigned)block), lds_bytes,
                        stream, args, classes, tiles);
     return hipGetLastError();

@@ -249,17 +249,28 @@ def test_autotuned_plan_matches_oracle(torch_dev, oracle):
 
 
 def test_launch_forms_match_oracle(torch_dev, oracle):
-    """The default single launch (every class at the largest register budget: classes
-    scheduled for a smaller budget run with idle butterfly slots) and the per-budget form
-    (ASDFFT_RAGGED_PER_BUDGET, one concurrent launch per budget) both meet the bound, C2C / R2C /
-    C2R, both directions, lengths from every budget."""
+    """Every launch form meets the bound, C2C / R2C / C2R, both directions, lengths from every
+    budget: the default single launch at the largest budget (classes scheduled for a smaller
+    budget run with idle butterfly slots), wide EMAX-16 workgroups of up to 1024 threads with the
+    classes that need a larger budget in one more launch (ASDFFT_RAGGED_WIDE16=1), and the
+    per-budget form (ASDFFT_RAGGED_PER_BUDGET)."""
     from sip_amd.fft import RaggedPlan
     rng = np.random.default_rng(18)
     spec = [("c2c", 64, 40, False), ("c2c", 4096, 3, True), ("r2c", 8192, 4, False), ("c2r", 1000, 5, True),
             ("c2c", 16384, 2, False), ("r2c", 243, 9, True), ("c2c", 14400, 2, True), ("c2r", 6000, 3, False)]
-    inputs = [make_input(rng, k, n, c) for k, n, c, _ in spec]
-    for per_budget, launches in ((False, "1 launch(es)"), (True, "3 launch(es)")):
-        plan = RaggedPlan(spec, per_budget=per_budget)
+    import os
+    inputs = [make_input(rng, k, n, c) for k, n, c, _ in spec + [("c2c", 15625, 2, False)]]
+    spec = spec + [("c2c", 15625, 2, False)]  # 5^6: no EMAX-16 schedule within 1024 threads
+    # default: one launch; wide EMAX-16 workgroups + one launch at EMAX 48 for 15625; per
+    # budget: EMAX 16 / 32 / 48
+    for wide, per_budget, launches in (("0", False, "1 launch(es)"), ("1", False, "2 launch(es)"),
+                                       (None, True, "3 launch(es)")):
+        if wide is not None:
+            os.environ["ASDFFT_RAGGED_WIDE16"] = wide
+        try:
+            plan = RaggedPlan(spec, per_budget=per_budget)
+        finally:
+            os.environ.pop("ASDFFT_RAGGED_WIDE16", None)
         try:
             assert launches in plan.describe(), plan.describe()
             outs = run_ragged(torch_dev, plan, inputs)

@@ -213,3 +213,46 @@ def test_serve_unknown_server(L):
     assert L.asdFftExtServeStats(fake, ctypes.byref(N.ServeCounters())) == INVALID
     assert L.asdFftExtServeDestroy(fake) == INVALID
     assert L.asdFftExtServeReplay(fake, None, None, 0, None) == INVALID
+
+
+def test_workload_schedules_fit_their_launch():
+    """Every class of the 4000-transform workload fits the workgroup of its launch (threads per
+    transform <= block): with wide EMAX-16 workgroups of up to 1024 threads (plus one launch for
+    the classes that need a larger register budget), in the default single launch, and in the
+    per-budget form."""
+    import collections
+    import os
+    import re
+
+    import bench
+    from sip_amd.fft import RaggedPlan
+    groups = collections.OrderedDict()
+    for k, n in sorted(bench.ragged_workload()):
+        groups[(k, n)] = groups.get((k, n), 0) + 1
+    entries = [(k, n, c, False) for (k, n), c in groups.items()]
+    forms = []
+    for wide, per_budget in (("1", False), ("0", False), (None, True)):
+        if wide is None:
+            os.environ.pop("ASDFFT_RAGGED_WIDE16", None)
+        else:
+            os.environ["ASDFFT_RAGGED_WIDE16"] = wide
+        try:
+            plan = RaggedPlan(entries, per_budget=per_budget)
+            text = plan.describe()
+            plan.close()
+        finally:
+            os.environ.pop("ASDFFT_RAGGED_WIDE16", None)
+        block = None
+        launches = 0
+        for line in text.splitlines():
+            m = re.search(r"launch emax=(\d+) block=(\d+)", line)
+            if m:
+                block = int(m.group(2))
+                launches += 1
+                assert block <= (1024 if m.group(1) == "16" else 512), line
+                continue
+            m = re.search(r" T=(\d+) tpb=(\d+)", line)
+            if m:
+                assert int(m.group(1)) * int(m.group(2)) <= block, line
+        forms.append(launches)
+    assert forms[0] <= 2 and forms[1] == 1 and forms[2] == 3, forms
