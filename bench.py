@@ -1003,6 +1003,7 @@ def main():
     ap.add_argument("--vendor-no-torch", action="store_true",
                     help="comparator form without the torch.fft path (direct hipFFT only; for kernel traces)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving form")
+    ap.add_argument("--precision-only", action="store_true", help="run only the precision form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device tensor copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
@@ -1022,6 +1023,9 @@ def main():
         return
     if a.serve_only:
         print(json.dumps({"forms": {"serving": bench_serve(torch, dev)}}))
+        return
+    if a.precision_only:
+        print(json.dumps({"forms": {"precision": bench_precision(torch, dev, stream, max(5, a.samples))}}))
         return
     if a.separated_only:
         print(json.dumps({"forms": {"separated": bench_separated(torch, dev, stream, max(5, a.samples))}}))

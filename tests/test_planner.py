@@ -70,14 +70,23 @@ def test_real_odd_uses_full_length():
     p.close()
 
 
-def test_multidim_plans_have_no_transposes():
+def test_multidim_plans_have_no_transposes(monkeypatch):
     p = Plan("c2c", (2048, 2048), 4)
     s = steps(p.describe())
-    assert len(s) == 2 and "[in->out]" in s[0] and "[out->out]" in s[1]
-    # strided column pass: the 2048-point columns exchange through one LDS plane (variant 15),
+    assert len(s) == 2 and "[in->out]" in s[0] and "[out->out]" in s[1] and p.workspace_size == 0
+    # paired rows (variant 21: rows n and n + 1024 with the columns' first radix-2 stage), then
+    # 1024-point columns on one LDS plane (variant 15), both in place on the output
+    assert field(s[0], "variant") == "21"
+    assert field(s[1], "variant") == "15" and field(s[1], "n") == "1024" and field(s[1], "lds") == "69632"
+    p.close()
+    monkeypatch.setenv("ASDFFT_PAIR2D", "0")
+    p = Plan("c2c", (2048, 2048), 4)
+    s = steps(p.describe())
+    # without pairing: the 2048-point columns exchange through one LDS plane (variant 15),
     # 16 columns per workgroup (128-byte runs) in 136 KiB
     assert field(s[1], "variant") == "15" and field(s[1], "tpb") == "16" and field(s[1], "lds") == "139264"
     p.close()
+    monkeypatch.delenv("ASDFFT_PAIR2D")
     p = Plan("c2c", (1024, 1024), 4)  # 1024-point columns: 16 per workgroup on one plane, two per CU
     s1 = steps(p.describe())[1]
     assert field(s1, "variant") == "15" and field(s1, "tpb") == "16" and field(s1, "lds") == "69632"
