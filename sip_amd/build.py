@@ -39,7 +39,7 @@ HOST_SOURCES = ["acl_tensor.cpp", "twiddles.cpp", "planner.cpp", "ragged.cpp", "
 KERNEL_HEADERS = ["stockham_impl.h", "fft_kernel_args.h", "kernels.h", "radix_consts.h", "ragged.h", "mfma_fft.h", "shift.h"]
 # hand-written kernel translation units besides the generated instances
 KERNEL_SOURCES = ["dft_direct.hip", "mfma_fft.hip", "shift_kernel.hip", "istft_kernel.hip", "real_aux.hip",
-                  "static_scratch.hip"]
+                  "static_scratch.hip", "wave_fft.hip"]
 # the ragged kernel builds once per register budget (one translation unit each)
 RAGGED_EMAX = [16, 32, 48]
 

@@ -18,7 +18,8 @@
 //
 // Two kernels share the row transform:
 //  * mfma_fft1024_kernel: contiguous rows in and out (a 1D plan of length 1024).  Each wave
-//    loops over rows and loads its next row while it transforms the current one.
+//    loops over rows with the next two rows' loads in flight while it transforms the current
+//    one; the A operands F (the same for every wave) are read from LDS per stage.
 //  * mfma_fft1024_tr_kernel: the second pass of a four-step transform whose output is stored
 //    transposed: row t = g * R + k (R rows per group) writes element m to
 //    out[g * R * 1024 + k + R * m].  A workgroup transforms a tile of 16 consecutive rows of
@@ -128,10 +129,9 @@ __device__ __forceinline__ void load_row(const float2* x, int lane, float2 (&v)[
 // The 1024-point transform of one row held in v (stage-1 B layout).  xpose_put(k1, n2, re, im)
 // and xpose_get(k1, n2, &re, &im) store and fetch the twiddled stage-1 result through LDS
 // (wave-private locations); put(m, re, im) receives output element m.
-template <typename XPut, typename XGet, typename Put>
-__device__ __forceinline__ void row_fft(float2 (&v)[2][8], const Split (&fr)[2], const Split (&fi)[2],
-                                        const float2* tw, int lane, float csign_in, float csign_out, XPut xpose_put,
-                                        XGet xpose_get, Put put)
+template <typename FGet, typename XPut, typename XGet, typename Put>
+__device__ __forceinline__ void row_fft(float2 (&v)[2][8], FGet fget, const float2* tw, int lane, float csign_in,
+                                        float csign_out, XPut xpose_put, XGet xpose_get, Put put)
 {
     float m = 0.0f;
     for (int c = 0; c < 2; c++) {
@@ -147,7 +147,11 @@ __device__ __forceinline__ void row_fft(float2 (&v)[2][8], const Split (&fr)[2],
     }
     // ---- stage 1
     f16acc yr = {}, yi = {};
-    for (int c = 0; c < 2; c++) cmatmul(fr[c], fi[c], xr[c], xi[c], yr, yi);
+    for (int c = 0; c < 2; c++) {
+        Split fr, fi;
+        fget(c, fr, fi);
+        cmatmul(fr, fi, xr[c], xi[c], yr, yi);
+    }
     // ---- twiddle (and 2^-5), then transpose through LDS
     for (int q = 0; q < 16; q++) {
         const int k1 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
@@ -169,7 +173,11 @@ __device__ __forceinline__ void row_fft(float2 (&v)[2][8], const Split (&fr)[2],
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before the locations are reused
     f16acc zr = {}, zi = {};
-    for (int c = 0; c < 2; c++) cmatmul(fr[c], fi[c], xr[c], xi[c], zr, zi);
+    for (int c = 0; c < 2; c++) {
+        Split fr, fi;
+        fget(c, fr, fi);
+        cmatmul(fr, fi, xr[c], xi[c], zr, zi);
+    }
     // ---- output: Z^T[k2][k1] -> X[k1 + 32 k2], k1 = lane % 32
     const float back = 32.0f / s;
     for (int q = 0; q < 16; q++) {
@@ -182,12 +190,25 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
 {
     __shared__ float2 tw[1024];              // W_1024^m
     __shared__ float tr[kWaves][2 * kPlane];  // per-wave transpose buffer (re plane, im plane)
+    __shared__ Split sf[2][2][64];            // A operands F[c] (real, imaginary) of each lane
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
-    Split fr[2], fi[2];
-    load_dft_matrix(a.w32, lane, fr, fi);
+    if (wave == 0) {
+        // the operands are the same for every wave: kept in LDS and read per stage, so they
+        // do not take registers for the whole kernel
+        Split fr[2], fi[2];
+        load_dft_matrix(a.w32, lane, fr, fi);
+        for (int c = 0; c < 2; c++) {
+            sf[c][0][lane] = fr[c];
+            sf[c][1][lane] = fi[c];
+        }
+    }
     __syncthreads();
+    auto fget = [&](int c, Split& fr, Split& fi) {
+        fr = sf[c][0][lane];
+        fi = sf[c][1][lane];
+    };
     const float csign_in = a.conj_in ? -1.0f : 1.0f;
     const float csign_out = a.conj_out ? -1.0f : 1.0f;
     float* tre = tr[wave];
@@ -202,17 +223,21 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
     };
     const int64_t rows_per_step = (int64_t)gridDim.x * kWaves;
     int64_t row = (int64_t)blockIdx.x * kWaves + wave;
-    float2 next[2][8];
+    // two rows ahead are in flight while a row is transformed
+    float2 next[2][8], next2[2][8];
     if (row < a.count) load_row(a.in + row * 1024, lane, next);
+    if (row + rows_per_step < a.count) load_row(a.in + (row + rows_per_step) * 1024, lane, next2);
     for (; row < a.count; row += rows_per_step) {
         float2 v[2][8];
         for (int c = 0; c < 2; c++) {
-            for (int j = 0; j < 8; j++) v[c][j] = next[c][j];
+            for (int j = 0; j < 8; j++) {
+                v[c][j] = next[c][j];
+                next[c][j] = next2[c][j];
+            }
         }
-        // the next row's loads are in flight while this row is transformed
-        if (row + rows_per_step < a.count) load_row(a.in + (row + rows_per_step) * 1024, lane, next);
+        if (row + 2 * rows_per_step < a.count) load_row(a.in + (row + 2 * rows_per_step) * 1024, lane, next2);
         float2* y = a.out + row * 1024;
-        row_fft(v, fr, fi, tw, lane, csign_in, csign_out, xput, xget,
+        row_fft(v, fget, tw, lane, csign_in, csign_out, xput, xget,
                 [&](int m, float re, float im) { y[m] = make_float2(re, im); });
     }
 }
@@ -225,9 +250,13 @@ __global__ void __launch_bounds__(64 * kTrWaves) mfma_fft1024_tr_kernel(const Mf
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
-    Split fr[2], fi[2];
-    load_dft_matrix(a.w32, lane, fr, fi);
+    Split frr[2], fir[2];
+    load_dft_matrix(a.w32, lane, frr, fir);
     __syncthreads();
+    auto fget = [&](int c, Split& fr, Split& fi) {
+        fr = frr[c];
+        fi = fir[c];
+    };
     const float csign_in = a.conj_in ? -1.0f : 1.0f;
     const float csign_out = a.conj_out ? -1.0f : 1.0f;
     // tile: rows t0 .. t0 + 15 of group g (R = out_rows rows per group, R % 16 == 0)
@@ -249,9 +278,9 @@ __global__ void __launch_bounds__(64 * kTrWaves) mfma_fft1024_tr_kernel(const Mf
         };
         auto put = [&](int m, float re, float im) { tile[tr_slot(m, r)] = make_float2(re, im); };
         if (h == 0) {
-            row_fft(v0, fr, fi, tw, lane, csign_in, csign_out, xput, xget, put);
+            row_fft(v0, fget, tw, lane, csign_in, csign_out, xput, xget, put);
         } else {
-            row_fft(v1, fr, fi, tw, lane, csign_in, csign_out, xput, xget, put);
+            row_fft(v1, fget, tw, lane, csign_in, csign_out, xput, xget, put);
         }
     }
     __syncthreads();

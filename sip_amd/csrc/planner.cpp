@@ -498,6 +498,46 @@ bool smooth_single(int64_t n) { return n > 1 && n <= kMaxSingleKernel && !choose
 // ---- 1D building blocks -------------------------------------------------------------
 // Contiguous complex rows of length n: count rows, in/out row distance n.  pin / pout:
 // the input / output is a pair of float planes (real, imaginary) instead of complex.
+// Lane-butterfly kernel for short C2C rows: ASDFFT_WAVE=1 wherever it has an instance, 0 never;
+// unset: never (the Stockham kernels measured faster at every length it covers).
+int wave_mode()
+{
+    const char* e = getenv("ASDFFT_WAVE");
+    if (e != nullptr && (e[0] == '0' || e[0] == '1')) return e[0] - '0';
+    return -1;
+}
+
+bool wave_wanted(int64_t n)
+{
+    int e = 0, lanes = 0;
+    return wave_mode() == 1 && wave_fft_shape(n, &e, &lanes);
+}
+
+// Contiguous C2C rows on the lane-butterfly kernel (wave_fft.hip).
+void wave_rows(int64_t n, int64_t count, bool inverse, Step* st)
+{
+    int e = 0, lanes = 0;
+    wave_fft_shape(n, &e, &lanes);
+    st->wave = true;
+    TablePtr w = upload_table("roots:" + std::to_string(n), roots_table(n, n));
+    st->tables_ready = (bool)w;
+    if (w) st->tables.push_back(w);
+    WaveFftArgs& a = st->wargs;
+    a.tw = w ? reinterpret_cast<const float2*>(w->dev) : nullptr;
+    a.count = count;
+    a.in_dist = n;
+    a.out_dist = n;
+    a.n = (int)n;
+    a.e = e;
+    a.lanes = lanes;
+    a.conj_in = inverse;
+    a.conj_out = inverse;
+    std::ostringstream os;
+    os << "wave n=" << n << " count=" << count << " points/lane=" << e << " lanes=" << lanes
+       << " (cross-lane radix-2 stages, no LDS)";
+    st->describe = os.str();
+}
+
 bool c2c_rows(int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPlan* bp, std::string* why,
               bool pin = false, bool pout = false)
 {
@@ -515,6 +555,8 @@ bool c2c_rows(int64_t n, int64_t count, bool inverse, int src, int dst, BuiltPla
         if (!direct_step(n, pin ? kProPlanar : kProComplex, pout ? kEpiPlanar : kEpiComplex, count, n, n, inverse,
                          inverse, &st, why))
             return false;
+    } else if (!pin && !pout && wave_wanted(n)) {
+        wave_rows(n, count, inverse, &st);
     } else {
         if (pout && !pin) {
             *why = "no row kernel with a complex input and a planar output";
@@ -1328,6 +1370,12 @@ hipError_t launch_step(const Step& s, char* const bufs[3], const float* in_im, f
         a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
         a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
         return launch_mfma_fft1024(a, s.mfma_grid, stream);
+    }
+    if (s.wave) {
+        WaveFftArgs a = s.wargs;
+        a.in = reinterpret_cast<const float2*>(bufs[s.src] + s.src_offset);
+        a.out = reinterpret_cast<float2*>(bufs[s.dst] + s.dst_offset);
+        return launch_wave_fft(a, stream);
     }
     if (s.aux) {
         RealAuxArgs a = s.raux;

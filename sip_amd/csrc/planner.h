@@ -18,6 +18,7 @@
 #include "kernels.h"
 #include "mfma_fft.h"
 #include "twiddles.h"
+#include "wave_fft.h"
 
 namespace asdfft {
 
@@ -67,6 +68,8 @@ struct Step {
     bool direct = false;
     bool mfma = false;       // split-FP16 MFMA kernel (margs, mfma_grid)
     bool aux = false;        // elementwise real pass (raux, real_aux.hip)
+    bool wave = false;       // lane-butterfly kernel for short rows (wargs, wave_fft.hip)
+    WaveFftArgs wargs{};
     RealAuxArgs raux{};
     MfmaFftArgs margs{};
     int64_t mfma_grid = 0;

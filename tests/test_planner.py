@@ -178,3 +178,28 @@ def test_column_spans_beyond_int32(make, n, stride):
         assert span == (n - 1) * stride > 2 ** 31
     finally:
         L.asdFftDestroy(h)
+
+
+def test_lane_butterfly_rows_are_opt_in(monkeypatch):
+    """ASDFFT_WAVE=1 puts contiguous C2C rows of n = e * lanes (e in {16, 12, 10, 9}, lanes a
+    power of two from 4 to 64) on the lane-butterfly kernel; unset or 0, and for every other
+    length or layout, the Stockham kernels keep them."""
+    want = {1024: (16, 64), 256: (16, 16), 64: (16, 4), 768: (12, 64), 640: (10, 64), 576: (9, 64), 36: (9, 4)}
+    for n, (e, lanes) in want.items():
+        p = Plan("c2c", (n,), 512)
+        assert "wave n=" not in p.describe()
+        p.close()
+        monkeypatch.setenv("ASDFFT_WAVE", "1")
+        p = Plan("c2c", (n,), 512)
+        s = steps(p.describe())
+        assert len(s) == 1 and f"wave n={n} count=512 points/lane={e} lanes={lanes}" in s[0], s
+        p.close()
+        monkeypatch.delenv("ASDFFT_WAVE")
+    monkeypatch.setenv("ASDFFT_WAVE", "1")
+    for n in (1000, 4096, 2048, 32, 15 * 64):  # no instance: 1000 = 8 x 125, 4096 / 16 = 256 lanes
+        p = Plan("c2c", (n,), 64)
+        assert "wave n=" not in p.describe(), n
+        p.close()
+    p = Plan("c2c", (1024,), 64, vertical=True)  # strided transforms stay on the column kernels
+    assert "wave n=" not in p.describe()
+    p.close()
