@@ -59,7 +59,9 @@ struct Split {
 
 // FP32 -> FP16 hi part (round to nearest even) and the FP16 rounding of the remainder.
 __device__ __forceinline__ _Float16 hi_of(float v) { return (_Float16)v; }
-__device__ __forceinline__ _Float16 lo_of(float v, _Float16 hi) { return (_Float16)(v - (float)hi); }
+// v - hi is exact in FP32; written as an FMA so that it maps onto the mixed-precision FMA
+// (f16 operand, f16 result) instead of a convert, a subtract and a convert.
+__device__ __forceinline__ _Float16 lo_of(float v, _Float16 hi) { return (_Float16)fmaf((float)hi, -1.0f, v); }
 
 __device__ __forceinline__ f16acc mfma(const half8& a, const half8& b, f16acc c)
 {
@@ -68,9 +70,29 @@ __device__ __forceinline__ f16acc mfma(const half8& a, const half8& b, f16acc c)
 
 __device__ __forceinline__ half8 neg8(half8 v) { return -v; }
 
-// Complex product accumulation: (re, im) += (Fr + i Fi) (Xr + i Xi), split three-term.
-__device__ __forceinline__ void cmatmul(const Split& fr, const Split& fi, const Split& xr, const Split& xi,
-                                        f16acc& re, f16acc& im)
+// Complex product accumulation: (re, im) += (Fr + i Fi) (Xr + i Xi), split three-term;
+// fin = -Fi (negating an FP16 operand is exact).
+__device__ __forceinline__ void cmatmul(const Split& fr, const Split& fi, const Split& fin, const Split& xr,
+                                        const Split& xi, f16acc& re, f16acc& im)
+{
+    re = mfma(fr.hi, xr.hi, re);
+    re = mfma(fr.hi, xr.lo, re);
+    re = mfma(fr.lo, xr.hi, re);
+    re = mfma(fin.hi, xi.hi, re);
+    re = mfma(fin.hi, xi.lo, re);
+    re = mfma(fin.lo, xi.hi, re);
+    im = mfma(fr.hi, xi.hi, im);
+    im = mfma(fr.hi, xi.lo, im);
+    im = mfma(fr.lo, xi.hi, im);
+    im = mfma(fi.hi, xr.hi, im);
+    im = mfma(fi.hi, xr.lo, im);
+    im = mfma(fi.lo, xr.hi, im);
+}
+
+// The same product with the negation on the B operand (for callers that hold F, not -Fi, in
+// registers).
+__device__ __forceinline__ void cmatmul_negx(const Split& fr, const Split& fi, const Split& xr, const Split& xi,
+                                             f16acc& re, f16acc& im)
 {
     const half8 xih_n = neg8(xi.hi), xil_n = neg8(xi.lo);
     re = mfma(fr.hi, xr.hi, re);
@@ -93,13 +115,39 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-__device__ __forceinline__ void split_into(float re, float im, Split& xr, Split& xi, int j)
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// hi and lo FP16 parts of the pair (a, b), each packed into one dword (a in the low half).
+// hi: one packed round-to-nearest convert.  lo = a - hi(a), b - hi(b): exact in FP32 and rounded
+// once to FP16 by the mixed-precision FMAs (f16 operand selected by op_sel, f32 operands),
+// two instructions for the pair.
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo)
 {
-    const _Float16 rh = hi_of(re), ih = hi_of(im);
-    xr.hi[j] = rh;
-    xr.lo[j] = lo_of(re, rh);
-    xi.hi[j] = ih;
-    xi.lo[j] = lo_of(im, ih);
+    const half2v h = {(_Float16)a, (_Float16)b};
+    hi = __builtin_bit_cast(uint32_t, h);
+    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel:[0,0,0] op_sel_hi:[1,0,0]\n\t"
+        "v_fma_mixhi_f16 %0, %1, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=&v"(lo)
+        : "v"(hi), "v"(a), "v"(b));
+}
+
+// Points j and j + 1 (j even) of the B operands xr / xi.
+__device__ __forceinline__ void split_into2(float re0, float im0, float re1, float im1, Split& xr, Split& xi, int j)
+{
+    u32x4 rh = __builtin_bit_cast(u32x4, xr.hi), rl = __builtin_bit_cast(u32x4, xr.lo);
+    u32x4 ih = __builtin_bit_cast(u32x4, xi.hi), il = __builtin_bit_cast(u32x4, xi.lo);
+    uint32_t h, l;
+    split_pair(re0, re1, h, l);
+    rh[j >> 1] = h;
+    rl[j >> 1] = l;
+    split_pair(im0, im1, h, l);
+    ih[j >> 1] = h;
+    il[j >> 1] = l;
+    xr.hi = __builtin_bit_cast(half8, rh);
+    xr.lo = __builtin_bit_cast(half8, rl);
+    xi.hi = __builtin_bit_cast(half8, ih);
+    xi.lo = __builtin_bit_cast(half8, il);
 }
 
 // A operands: F[i][k] = W_32^(i k) for i = lane % 32, k = 16 c + 8 (lane / 32) + j
@@ -119,18 +167,25 @@ __device__ __forceinline__ void load_dft_matrix(const float2* w32, int lane, Spl
 }
 
 // Stage-1 B operand of one row: B[n1][n2] = x[32 n1 + n2], n2 = lane % 32 (256-byte runs).
+// The lane's base pointer plus compile-time offsets below 4 KiB: one address per 16 loads
+// of each half instead of one per load.
 __device__ __forceinline__ void load_row(const float2* x, int lane, float2 (&v)[2][8])
 {
+    const float2* p = x + 256 * (lane >> 5) + (lane & 31);
     for (int c = 0; c < 2; c++) {
-        for (int j = 0; j < 8; j++) v[c][j] = x[32 * (16 * c + 8 * (lane >> 5) + j) + (lane & 31)];
+        const float2* pc = p + 512 * c;
+        for (int j = 0; j < 8; j++) v[c][j] = pc[32 * j];
     }
 }
 
 // The 1024-point transform of one row held in v (stage-1 B layout).  xpose_put(k1, n2, re, im)
 // and xpose_get(k1, n2, &re, &im) store and fetch the twiddled stage-1 result through LDS
-// (wave-private locations); put(m, re, im) receives output element m.
-template <typename FGet, typename XPut, typename XGet, typename Put>
-__device__ __forceinline__ void row_fft(float2 (&v)[2][8], FGet fget, const float2* tw, int lane, float csign_in,
+// (wave-private locations); put(mq, re, im) receives output element m = mlane + mq with
+// mlane = (lane % 32) + 128 (lane / 32) and mq a compile-time constant per call.  tw holds
+// W_1024^m / 32 (the 2^-5 between the stages folded in).  mm(c, xr, xi, re, im) accumulates
+// (re, im) += F[c] (xr + i xi) for the A operand block c of the DFT matrix.
+template <typename MM, typename XPut, typename XGet, typename Put>
+__device__ __forceinline__ void row_fft(float2 (&v)[2][8], MM mm, const float2* tw, int lane, float csign_in,
                                         float csign_out, XPut xpose_put, XGet xpose_get, Put put)
 {
     float m = 0.0f;
@@ -143,46 +198,42 @@ __device__ __forceinline__ void row_fft(float2 (&v)[2][8], FGet fget, const floa
     const float s = ldexpf(1.0f, 9 - e);
     Split xr[2], xi[2];
     for (int c = 0; c < 2; c++) {
-        for (int j = 0; j < 8; j++) split_into(v[c][j].x * s, v[c][j].y * s * csign_in, xr[c], xi[c], j);
+        for (int j = 0; j < 8; j += 2) {
+            split_into2(v[c][j].x * s, v[c][j].y * s * csign_in, v[c][j + 1].x * s, v[c][j + 1].y * s * csign_in,
+                        xr[c], xi[c], j);
+        }
     }
     // ---- stage 1
     f16acc yr = {}, yi = {};
-    for (int c = 0; c < 2; c++) {
-        Split fr, fi;
-        fget(c, fr, fi);
-        cmatmul(fr, fi, xr[c], xi[c], yr, yi);
-    }
-    // ---- twiddle (and 2^-5), then transpose through LDS
+    for (int c = 0; c < 2; c++) mm(c, xr[c], xi[c], yr, yi);
+    // ---- twiddle (tw carries the 2^-5), then transpose through LDS
     for (int q = 0; q < 16; q++) {
         const int k1 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
         const int n2 = lane & 31;
         const float2 w = tw[(k1 * n2) & 1023];
         const float re = yr[q] * w.x - yi[q] * w.y;
         const float im = yr[q] * w.y + yi[q] * w.x;
-        xpose_put(k1, n2, re * 0.03125f, im * 0.03125f);
+        xpose_put(k1, n2, re, im);
     }
     // this wave's transpose writes complete before its lanes read other lanes' values
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     // ---- stage 2 B operand: B[n2][k1] = Y'[k1][n2], k1 = lane % 32, n2 = 16 c + 8 (lane / 32) + j
     for (int c = 0; c < 2; c++) {
-        for (int j = 0; j < 8; j++) {
-            float re, im;
-            xpose_get(lane & 31, 16 * c + 8 * (lane >> 5) + j, &re, &im);
-            split_into(re, im, xr[c], xi[c], j);
+        for (int j = 0; j < 8; j += 2) {
+            float re0, im0, re1, im1;
+            xpose_get(lane & 31, 16 * c + 8 * (lane >> 5) + j, &re0, &im0);
+            xpose_get(lane & 31, 16 * c + 8 * (lane >> 5) + j + 1, &re1, &im1);
+            split_into2(re0, im0, re1, im1, xr[c], xi[c], j);
         }
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before the locations are reused
     f16acc zr = {}, zi = {};
-    for (int c = 0; c < 2; c++) {
-        Split fr, fi;
-        fget(c, fr, fi);
-        cmatmul(fr, fi, xr[c], xi[c], zr, zi);
-    }
+    for (int c = 0; c < 2; c++) mm(c, xr[c], xi[c], zr, zi);
     // ---- output: Z^T[k2][k1] -> X[k1 + 32 k2], k1 = lane % 32
     const float back = 32.0f / s;
     for (int q = 0; q < 16; q++) {
-        const int k2 = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        put((lane & 31) + 32 * k2, zr[q] * back, zi[q] * back * csign_out);
+        // k2 = 8 (q / 4) + 4 (lane / 32) + q % 4; m = (lane % 32) + 32 k2
+        put(32 * (8 * (q >> 2) + (q & 3)), zr[q] * back, zi[q] * back * csign_out);
     }
 }
 
@@ -190,10 +241,13 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
 {
     __shared__ float2 tw[1024];              // W_1024^m
     __shared__ float tr[kWaves][2 * kPlane];  // per-wave transpose buffer (re plane, im plane)
-    __shared__ Split sf[2][2][64];            // A operands F[c] (real, imaginary) of each lane
+    __shared__ Split sf[2][3][64];            // A operands F[c] (real, imaginary, -imaginary) per lane
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
+    for (int m = threadIdx.x; m < 1024; m += blockDim.x) {
+        const float2 w = a.tw1024[m];
+        tw[m] = make_float2(w.x * 0.03125f, w.y * 0.03125f);  // exact: a power of two
+    }
     if (wave == 0) {
         // the operands are the same for every wave: kept in LDS and read per stage, so they
         // do not take registers for the whole kernel
@@ -202,12 +256,16 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
         for (int c = 0; c < 2; c++) {
             sf[c][0][lane] = fr[c];
             sf[c][1][lane] = fi[c];
+            sf[c][2][lane] = Split{neg8(fi[c].hi), neg8(fi[c].lo)};
         }
     }
     __syncthreads();
-    auto fget = [&](int c, Split& fr, Split& fi) {
-        fr = sf[c][0][lane];
-        fi = sf[c][1][lane];
+    auto mm = [&](int c, const Split& xr, const Split& xi, f16acc& re, f16acc& im) {
+        // read at every use: the barrier keeps the compiler from hoisting the loop-invariant
+        // reads out of the row loop and holding all of F in registers again
+        asm volatile("" ::: "memory");
+        const Split fr = sf[c][0][lane], fi = sf[c][1][lane], fin = sf[c][2][lane];
+        cmatmul(fr, fi, fin, xr, xi, re, im);
     };
     const float csign_in = a.conj_in ? -1.0f : 1.0f;
     const float csign_out = a.conj_out ? -1.0f : 1.0f;
@@ -236,9 +294,9 @@ __global__ void __launch_bounds__(64 * kWaves) mfma_fft1024_kernel(const MfmaFft
             }
         }
         if (row + 2 * rows_per_step < a.count) load_row(a.in + (row + 2 * rows_per_step) * 1024, lane, next2);
-        float2* y = a.out + row * 1024;
-        row_fft(v, fget, tw, lane, csign_in, csign_out, xput, xget,
-                [&](int m, float re, float im) { y[m] = make_float2(re, im); });
+        float2* y = a.out + row * 1024 + (lane & 31) + 128 * (lane >> 5);
+        row_fft(v, mm, tw, lane, csign_in, csign_out, xput, xget,
+                [&](int mq, float re, float im) { y[mq] = make_float2(re, im); });
     }
 }
 
@@ -249,13 +307,15 @@ __global__ void __launch_bounds__(64 * kTrWaves) mfma_fft1024_tr_kernel(const Mf
     float2* tw = lds + kTrSlots;      // W_1024^m
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    for (int m = threadIdx.x; m < 1024; m += blockDim.x) tw[m] = a.tw1024[m];
+    for (int m = threadIdx.x; m < 1024; m += blockDim.x) {
+        const float2 w = a.tw1024[m];
+        tw[m] = make_float2(w.x * 0.03125f, w.y * 0.03125f);  // exact: a power of two
+    }
     Split frr[2], fir[2];
     load_dft_matrix(a.w32, lane, frr, fir);
     __syncthreads();
-    auto fget = [&](int c, Split& fr, Split& fi) {
-        fr = frr[c];
-        fi = fir[c];
+    auto mm = [&](int c, const Split& xr, const Split& xi, f16acc& re, f16acc& im) {
+        cmatmul_negx(frr[c], fir[c], xr, xi, re, im);
     };
     const float csign_in = a.conj_in ? -1.0f : 1.0f;
     const float csign_out = a.conj_out ? -1.0f : 1.0f;
@@ -276,11 +336,13 @@ __global__ void __launch_bounds__(64 * kTrWaves) mfma_fft1024_tr_kernel(const Mf
             *re = x.x;
             *im = x.y;
         };
-        auto put = [&](int m, float re, float im) { tile[tr_slot(m, r)] = make_float2(re, im); };
+        auto put = [&](int mq, float re, float im) {
+            tile[tr_slot((lane & 31) + 128 * (lane >> 5) + mq, r)] = make_float2(re, im);
+        };
         if (h == 0) {
-            row_fft(v0, fget, tw, lane, csign_in, csign_out, xput, xget, put);
+            row_fft(v0, mm, tw, lane, csign_in, csign_out, xput, xget, put);
         } else {
-            row_fft(v1, fget, tw, lane, csign_in, csign_out, xput, xget, put);
+            row_fft(v1, mm, tw, lane, csign_in, csign_out, xput, xget, put);
         }
     }
     __syncthreads();
