@@ -962,9 +962,13 @@ bool pair2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
         tab.push_back((float)(-sinl(ang)));
     }
     TablePtr pt = upload_table("pair2d:" + std::to_string(X), tab);
-    // rows n and n + X/2 of a slice are read and rewritten by one workgroup, and each column
-    // workgroup reads and rewrites only its own columns: both launches work on the output, in
-    // place, and the plan needs no workspace
+    // The row pass reads rows n and n + X/2 of the input and writes their sum and twiddled
+    // difference to rows 2n and 2n + 1 of the output.  Each column transform (parity p, column c)
+    // then reads and rewrites exactly the elements (2i + p, c), i < X/2, so the column pass is in
+    // place on the output with identical input and output addresses: no workgroup can overwrite
+    // what another has still to read, and the plan needs no workspace.  (Writing the halves to
+    // rows n and n + X/2 and interleaving them in the column pass raced: the parity-0 workgroup
+    // of a column writes rows that the parity-1 workgroup of the same column reads.)
     Step rows;
     rows.src = kBufIn;
     rows.dst = kBufOut;
@@ -975,8 +979,8 @@ bool pair2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
     ri.in_inner = h * Y;
     ri.in_outer = Y;
     ri.in_outer2 = X * Y;
-    ri.out_inner = h * Y;
-    ri.out_outer = Y;
+    ri.out_inner = Y;
+    ri.out_outer = 2 * Y;
     ri.out_outer2 = X * Y;
     ri.o_split = h;
     if (!stockham_step(Y, kVarRowsPair2, false, ri, inv, 0, 2, &rows, &why) || rows.shape.static_id < 0) return false;
@@ -991,9 +995,9 @@ bool pair2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
     ci.count = B * 2 * Y;
     ci.cols = Y;
     ci.in_inner = 1;
-    ci.in_outer = h * Y;
+    ci.in_outer = Y;
     ci.in_outer2 = X * Y;
-    ci.in_stride = Y;
+    ci.in_stride = 2 * Y;
     ci.out_inner = 1;
     ci.out_outer = Y;
     ci.out_outer2 = X * Y;

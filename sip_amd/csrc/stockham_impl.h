@@ -673,8 +673,9 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
     } else if constexpr (EPI == kEpiPairRadix2) {
         // rows mapping, two transforms per workgroup: 2 blk (row n) and 2 blk + 1 (row n + X/2
         // of the same slice), n = blk % o_split.  First radix-2 stage of the length-X column
-        // transforms (decimation in frequency): row n <- u + v, row n + X/2 <- (u - v) W_X^n,
-        // W_X^n = tw_aux[n] (forward sign; the inverse comes from the conjugations)
+        // transforms (decimation in frequency): u + v is stored at n out_outer + slice out_outer2
+        // and (u - v) W_X^n out_inner further on (the planner interleaves them as rows 2n and
+        // 2n + 1), W_X^n = tw_aux[n] (forward sign; the inverse comes from the conjugations)
         const int64_t t0 = blk * 2;
         if (t0 + 1 < a.count) {
             const float2 w = ldg2(a.tw_aux, blk % a.o_split);

@@ -203,3 +203,27 @@ def test_lane_butterfly_rows_are_opt_in(monkeypatch):
     p = Plan("c2c", (1024,), 64, vertical=True)  # strided transforms stay on the column kernels
     assert "wave n=" not in p.describe()
     p.close()
+
+
+def test_in_place_steps_read_and_write_the_same_elements():
+    """A step whose source and destination are the same buffer must address its input and its
+    output identically (stride and span), or one workgroup could overwrite what another has yet
+    to read.  The paired 2D plans once read their column inputs at stride Y and wrote at 2Y in
+    place, and raced on the GPU."""
+    shapes = [("c2c", (2048, 2048), 4), ("c2c", (1024, 1024), 16), ("c2c", (2048, 1024), 2), ("c2c", (1000, 1000), 1),
+              ("c2c", (64, 96, 80), 2), ("c2c", (256, 256, 256), 1), ("c2c", (512, 512, 8), 3), ("r2c", (2048, 2048), 1),
+              ("c2r", (512, 1024), 2), ("r2c", (64, 64, 64), 2), ("c2c", (65536,), 8), ("c2c", (131072,), 2)]
+    checked = 0
+    for kind, dims, batch in shapes:
+        p = Plan(kind, dims, batch)
+        for line in steps(p.describe()):
+            m = re.match(r"\[(\w+)->(\w+)\]", line)
+            if m is None or m.group(1) != m.group(2):
+                continue
+            st = re.search(r"stride=(\d+)/(\d+) span=(\d+)/(\d+)", line)
+            if st is None:
+                continue
+            checked += 1
+            assert st.group(1) == st.group(2) and st.group(3) == st.group(4), (kind, dims, batch, line)
+        p.close()
+    assert checked >= 4
