@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -72,9 +73,12 @@ struct Slot {
     char* d_stage = nullptr;
     char* h_stage = nullptr;
     // plain plans of a batch run one after another on the slot's stream: one workspace,
-    // grown to the largest plan seen
+    // grown to the largest workspace of any plan submitted so far.  A workspace that is
+    // outgrown is kept (and counted as resident) until the server closes: hipFree waits for
+    // the whole device, which stalled every stream for about a millisecond per growth
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
+    std::vector<void*> retired;
     size_t bytes = 0;
     bool busy = false;
     bool launched = false;
@@ -139,6 +143,8 @@ public:
     std::vector<TablePtr> tables;
     std::mutex plan_mu;  // plain plans are created by submitters and read by the scheduler
     std::map<std::string, std::unique_ptr<PlainPlan>> plans;
+    std::atomic<size_t> workspace_max{0};  // largest workspace of any plan built so far
+    int min_ragged = 2;  // small requests in a batch from which they share one ragged launch
 
     ~Server();
     bool init(std::string* why);
@@ -157,6 +163,8 @@ bool Server::init(std::string* why)
     }
     int n = 0;
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n > 0) cus = n;
+    // ASDFFT_SERVE_MIN_RAGGED=<k>: batches share a ragged launch from k small requests (measurement)
+    if (const char* e = getenv("ASDFFT_SERVE_MIN_RAGGED")) min_ragged = std::max(2, atoi(e));
     if (hipStreamCreateWithFlags(&setup, hipStreamNonBlocking) != hipSuccess) {
         *why = "stream creation failed";
         return false;
@@ -228,6 +236,7 @@ Server::~Server()
         if (s->d_stage) (void)hipFree(s->d_stage);
         if (s->h_stage) (void)hipHostFree(s->h_stage);
         if (s->workspace) (void)hipFree(s->workspace);
+        for (void* w : s->retired) (void)hipFree(w);
     }
     if (d_pool) (void)hipFree(d_pool);
     if (setup) (void)hipStreamDestroy(setup);
@@ -295,6 +304,9 @@ PlainPlan* Server::plain_plan(const Req& q, std::string* why)
         }
     }
     PlainPlan* raw = p.get();
+    size_t seen = workspace_max.load();
+    while (raw->built.workspace > seen && !workspace_max.compare_exchange_weak(seen, raw->built.workspace)) {
+    }
     plans[q.plain_key] = std::move(p);
     return raw;
 }
@@ -308,12 +320,12 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
     size_t ragged_bytes = 0;
     s.groups.clear();
     s.group_bytes.clear();
-    // the ragged launch pays a work-table copy: it runs when two or more small 1D requests
-    // share the batch; a lone request takes the plain exec path
+    // the ragged launch pays a work-table copy: it runs when at least min_ragged small 1D
+    // requests share the batch; fewer run back to back through their plain plans
     int eligible = 0;
     for (Req* q : batch) eligible += (q->ragged && q->bytes <= kRaggedMaxRequestBytes) ? 1 : 0;
     for (Req* q : batch) {
-        const bool take = eligible >= 2 && q->ragged && q->bytes <= kRaggedMaxRequestBytes;
+        const bool take = eligible >= min_ragged && q->ragged && q->bytes <= kRaggedMaxRequestBytes;
         int ci = take ? pool_class(q->entry) : -1;
         if (ci < 0) {
             plain.push_back(q);
@@ -413,20 +425,16 @@ int Server::launch_batch(Slot& s, int slot_index, const std::vector<Req*>& batch
         ws_need = std::max(ws_need, p->built.workspace);
     }
     if (ws_need > s.workspace_bytes) {
-        // the slot is idle (its previous batch completed), so its workspace is free to replace
-        if (s.workspace) (void)hipFree(s.workspace);
-        s.workspace = nullptr;
-        const size_t old_bytes = s.workspace_bytes;
-        s.workspace_bytes = 0;
-        if (hipMalloc(&s.workspace, ws_need) != hipSuccess) {
-            s.workspace = nullptr;
-            std::lock_guard<std::mutex> lock(mu);
-            resident -= old_bytes;
-            return 3;
-        }
-        s.workspace_bytes = ws_need;
+        // grow straight to the largest workspace submitted so far (one growth per slot while
+        // the request mix stays the same); the old one stays allocated, see Slot::retired
+        const size_t want = std::max(ws_need, workspace_max.load());
+        void* fresh = nullptr;
+        if (hipMalloc(&fresh, want) != hipSuccess) return 3;
+        if (s.workspace) s.retired.push_back(s.workspace);
+        s.workspace = fresh;
+        s.workspace_bytes = want;
         std::lock_guard<std::mutex> lock(mu);
-        resident += ws_need - old_bytes;
+        resident += want;
     }
     (void)slot_index;
     for (size_t k = 0; k < plain.size(); k++) {
