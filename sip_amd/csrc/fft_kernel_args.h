@@ -47,6 +47,7 @@ enum Epilogue : int {
     kEpiRowsOut = 8,   // column mapping: each transform stored contiguously through LDS
     kEpiShift = 9,     // complex store through the output shift (roll_out, out_col_roll), times out_scale
     kEpiPairRadix2 = 10,  // rows n and n + o_split of a pair, through LDS: u + v and (u - v) W^n (tw_aux)
+    kEpiQuadRadix4 = 11,  // rows n + q o_split, q < 4, through LDS: their 4-point DFT, bin k times W^(n k) (tw_aux)
 };
 
 struct PassDesc {

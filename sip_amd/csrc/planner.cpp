@@ -224,7 +224,7 @@ int split_variant_of(int variant)
 }
 
 // Column mapping: lanes across adjacent transforms (columns) for coalesced strided access.
-bool cols_config(int64_t n, int64_t count, int variant, Config* c)
+bool cols_config(int64_t n, int64_t count, int variant, Config* c, bool prefer_split = false)
 {
     std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
@@ -292,13 +292,14 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c)
     const int split_tpb_forced = st ? atoi(st) : 0;
     const int split_variant = split_variant_of(variant);
     const size_t two_per_cu = kLdsLimit / 2;
-    if (split_variant >= 0 && found && c->lds > two_per_cu && split_only != 0 && !(ns != nullptr && ns[0] == '1')) {
+    if (split_variant >= 0 && found && (c->lds > two_per_cu || prefer_split) && split_only != 0 &&
+        !(ns != nullptr && ns[0] == '1')) {
         bool done = false;
         for (int tpb : {16, 8}) {
             if (split_tpb_forced > 0 && tpb != split_tpb_forced) continue;
             const size_t lds = (size_t)n * (tpb + 1) * 4;
             if (count < tpb || lds > kLdsLimit) continue;
-            for (int emax : {32, 64}) {
+            for (int emax : {16, 32, 64}) {
                 if (split_only > 0 && emax != split_only) continue;
                 const int T = threads_for(n, radices, emax);
                 if (T == 0 || tpb * T > 1024) continue;
@@ -358,6 +359,7 @@ struct StepIo {
     int64_t in_inner = 0, in_outer = 0, in_stride = 1;
     int64_t out_inner = 0, out_outer = 0, out_stride = 1;
     int64_t o_split = 0, in_outer2 = 0, out_outer2 = 0;  // split outer index (FftArgs::o_split)
+    bool prefer_split = false;  // columns: the one-plane kernel even where two planes fit twice per CU
 };
 
 bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, int conj_in, int conj_out,
@@ -365,17 +367,23 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
 {
     Config c;
     const bool pair = (variant == kVarRowsPair2);
+    const bool quad = (variant == kVarRowsQuad4);
     const bool epi_lds = (variant == kVarR2CEven || variant == kVarRowsTranspose || variant == kVarRowsTransposeSep ||
-                          pair);
-    // the pair variant has the row C2C kernel's geometry, two transforms per workgroup
-    const bool ok = cols_mapping ? cols_config(n, io.count, variant, &c)
-                                 : rows_config(n, io.count, epi_lds, pair ? 2 : want_tpb,
-                                               pair ? (int)kVarRowsC2C : variant, &c);
-    if (!ok || (pair && (c.tpb != 2 || c.radices.size() < 2))) {
+                          pair || quad);
+    // the pair and quad variants have the row C2C kernel's geometry with two / four transforms
+    // per workgroup (a quad workgroup may exceed the C2C kernel's block: its instances fix it)
+    const bool ok = cols_mapping ? cols_config(n, io.count, variant, &c, io.prefer_split)
+                                 : rows_config(n, io.count, epi_lds, pair ? 2 : quad ? 1 : want_tpb,
+                                               (pair || quad) ? (int)kVarRowsC2C : variant, &c);
+    if (!ok || (pair && (c.tpb != 2 || c.radices.size() < 2)) || (quad && (c.emax != 16 || c.radices.size() < 2))) {
         *why = "no single-workgroup schedule for length " + std::to_string(n);
         return false;
     }
-    if (pair) {
+    if (quad) {
+        c.tpb = 4;
+        c.plane = 4 * c.tstride;
+    }
+    if (pair || quad) {
         c.split = false;
         c.lds = (size_t)(8 * c.tpb * c.tstride);
     }
@@ -1012,6 +1020,82 @@ bool pair2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
     return true;
 }
 
+// 2D C2C in two launches with the column transforms' first radix-4 stage in the row pass:
+// ASDFFT_QUAD2D=0 never, 1 wherever the geometry allows; unset, where both launches have
+// specialised kernels (the half-plane exchange column kernel included).
+int quad2d_mode()
+{
+    const char* e = getenv("ASDFFT_QUAD2D");
+    if (e != nullptr && (e[0] == '0' || e[0] == '1')) return e[0] - '0';
+    return -1;
+}
+
+bool quad2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
+{
+    if (quad2d_mode() == 0 || X % 4 != 0 || X / 4 < 2 || Y < 2) return false;
+    const int64_t q = X / 4;
+    std::string why;
+    // W_X^m, m < 3X/4 (the row pass multiplies bin k of row n by W_X^(n k)), forward sign
+    std::vector<float> tab;
+    static const long double kTwoPi = 6.283185307179586476925286766559005768L;
+    for (int64_t m = 0; m < 3 * q; m++) {
+        const long double ang = kTwoPi * (long double)m / (long double)X;
+        tab.push_back((float)cosl(ang));
+        tab.push_back((float)(-sinl(ang)));
+    }
+    TablePtr pt = upload_table("quad2d:" + std::to_string(X), tab);
+    // The row pass reads rows n + q X/4 (q < 4) of the input and writes bin k of their 4-point
+    // DFT, twiddled, to row 4n + k of the output.  Each column transform (k, column c) then
+    // reads and rewrites exactly the elements (4i + k, c), i < X/4: in place on the output with
+    // identical input and output addresses, no workspace.
+    Step rows;
+    rows.src = kBufIn;
+    rows.dst = kBufOut;
+    rows.bytes = 16.0 * (double)(X * Y * B);
+    StepIo ri;
+    ri.count = B * X;
+    ri.cols = 4;
+    ri.in_inner = q * Y;
+    ri.in_outer = Y;
+    ri.in_outer2 = X * Y;
+    ri.out_inner = Y;
+    ri.out_outer = 4 * Y;
+    ri.out_outer2 = X * Y;
+    ri.o_split = q;
+    if (!stockham_step(Y, kVarRowsQuad4, false, ri, inv, 0, 4, &rows, &why) ||
+        (rows.shape.static_id < 0 && quad2d_mode() != 1)) {
+        return false;
+    }
+    if (rows.shape.static_id < 0) return false;  // the four-row workgroup exists only as an instance
+    rows.args.tw_aux = pt ? pt->dev : nullptr;
+    rows.tables_ready = rows.tables_ready && pt != nullptr;
+    if (pt) rows.tables.push_back(pt);
+    Step cols;
+    cols.src = kBufOut;
+    cols.dst = kBufOut;
+    cols.bytes = 16.0 * (double)(X * Y * B);
+    StepIo ci;
+    ci.count = B * 4 * Y;
+    ci.cols = Y;
+    ci.in_inner = 1;
+    ci.in_outer = Y;
+    ci.in_outer2 = X * Y;
+    ci.in_stride = 4 * Y;
+    ci.out_inner = 1;
+    ci.out_outer = Y;
+    ci.out_outer2 = X * Y;
+    ci.out_stride = 4 * Y;
+    ci.o_split = 4;
+    // the quarter-length columns through one LDS plane: 16 columns (128-byte runs) in half the
+    // LDS of the two-plane kernel, so twice the workgroups fit a CU
+    ci.prefer_split = true;
+    if (!stockham_step(q, kVarColsC2C, true, ci, 0, inv, 0, &cols, &why)) return false;
+    if (quad2d_mode() != 1 && cols.shape.variant != kVarColsC2CSplit) return false;
+    out->steps.push_back(std::move(rows));
+    out->steps.push_back(std::move(cols));
+    return true;
+}
+
 }  // namespace
 
 static std::vector<int> search_radices(int64_t n)
@@ -1301,6 +1385,7 @@ std::string build_plan(const PlanSpec& spec, BuiltPlan* out)
 
     if (spec.kind == Kind::C2C) {
         if (L > kMaxSingleKernel) return "last-axis length above " + std::to_string(kMaxSingleKernel);
+        if (nd == 2 && quad2d(spec.dims[0], L, B, inv, out)) return "";
         if (nd == 2 && pair2d_wanted(spec.dims[0], L) && pair2d(spec.dims[0], L, B, inv, out)) return "";
         if (!c2c_rows(L, rows, inv, kBufIn, kBufOut, out, &why)) return why;
         for (size_t d = nd - 1; d-- > 0;) {

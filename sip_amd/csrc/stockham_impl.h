@@ -400,7 +400,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     constexpr int B = EMAX / R;
     static_assert(B >= 1, "radix larger than the per-thread register budget");
     constexpr bool kEpiViaLds = (EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar ||
-                                 EPI == kEpiRowsOut || EPI == kEpiPairRadix2);
+                                 EPI == kEpiRowsOut || EPI == kEpiPairRadix2 || EPI == kEpiQuadRadix4);
     if constexpr (RT) {
         asm volatile("" : "+s"(g.n), "+s"(g.T));
         asm volatile("" : "+v"(tid));
@@ -690,6 +690,33 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                 out1[i] = conj_if(cmul(csub(u, v), w), io.cout);
             }
         }
+    } else if constexpr (EPI == kEpiQuadRadix4) {
+        // rows mapping, four transforms per workgroup: 4 blk + q is row n + q X/4 of one slice,
+        // n = blk % o_split.  First radix-4 stage of the length-X column transforms (decimation
+        // in frequency): y_k = sum_q u_q W_4^(q k), stored times W_X^(n k) (tw_aux[n k]) at
+        // n out_outer + slice out_outer2 + k out_inner (the planner makes that row 4n + k)
+        const int64_t t0 = blk * 4;
+        if (t0 + 3 < a.count) {
+            const int64_t nn = blk % a.o_split;
+            const float2 w1 = ldg2(a.tw_aux, nn), w2 = ldg2(a.tw_aux, 2 * nn), w3 = ldg2(a.tw_aux, 3 * nn);
+            const int64_t ob = nn * a.out_outer + (blk / a.o_split) * a.out_outer2;
+            float2* o0 = reinterpret_cast<float2*>(a.out) + ob;
+            float2* o1 = o0 + a.out_inner;
+            float2* o2 = o1 + a.out_inner;
+            float2* o3 = o2 + a.out_inner;
+            for (int i = threadIdx.x; i < g.n; i += g.T * 4) {
+                const int at0 = lds_at<MAP>(g, 0, i), at1 = lds_at<MAP>(g, 1, i);
+                const int at2 = lds_at<MAP>(g, 2, i), at3 = lds_at<MAP>(g, 3, i);
+                const float2 u0 = make_float2(lre[at0], lim[at0]), u1 = make_float2(lre[at1], lim[at1]);
+                const float2 u2 = make_float2(lre[at2], lim[at2]), u3 = make_float2(lre[at3], lim[at3]);
+                const float2 a0 = cadd(u0, u2), a1 = csub(u0, u2);
+                const float2 b0 = cadd(u1, u3), b1 = mul_mi(csub(u1, u3));
+                o0[i] = conj_if(cadd(a0, b0), io.cout);
+                o1[i] = conj_if(cmul(cadd(a1, b1), w1), io.cout);
+                o2[i] = conj_if(cmul(csub(a0, b0), w2), io.cout);
+                o3[i] = conj_if(cmul(csub(a1, b1), w3), io.cout);
+            }
+        }
     } else if constexpr (EPI == kEpiRowsOut) {
         // column mapping: LDS holds the workgroup's tpb transforms interleaved; each is written
         // as one contiguous row (out_stride 1), consecutive lanes on consecutive elements
@@ -787,7 +814,8 @@ __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, fl
 template <int EMAX, int MAP, int T, int TPB>
 constexpr int static_block_bound()
 {
-    if constexpr (MAP == kMapCols) return T * TPB;
+    // columns, and row instances with a fixed number of transforms per workgroup
+    if constexpr (MAP == kMapCols || TPB > 1) return T * TPB;
     return T > max_block<EMAX>() ? (T + 63) / 64 * 64 : max_block<EMAX>();
 }
 
@@ -1036,6 +1064,7 @@ template <> struct VariantTraits<kVarRowsShiftBoth> { static constexpr int map =
 template <> struct VariantTraits<kVarColsShiftOut> { static constexpr int map = kMapCols, pro = kProComplex, epi = kEpiShift; };
 template <> struct VariantTraits<kVarRowsPair2> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiPairRadix2; };
 template <> struct VariantTraits<kVarRowsC2CTwPre> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiComplex; };
+template <> struct VariantTraits<kVarRowsQuad4> { static constexpr int map = kMapRows, pro = kProComplex, epi = kEpiQuadRadix4; };
 
 // One instantiation per kernel, so the LDS limit is raised once per kernel and device.
 template <auto Kern>

@@ -369,8 +369,9 @@ def test_opt_in_one_plane_kernels(torch_dev, oracle, monkeypatch, env, dims, bat
         assert err <= tol_for(dims), (env, dims, inverse, err)
 
 
-# 2D C2C with the columns' first radix-2 stage folded into the row pass (ASDFFT_PAIR2D=1):
-# rows n and n + X/2 in one workgroup, then length-X/2 columns stored to rows 2k + h
+# 2D C2C with the columns' first radix-2 stage folded into the row pass (ASDFFT_PAIR2D=1, the
+# radix-4 form off): rows n and n + X/2 in one workgroup written to rows 2n and 2n + 1, then the
+# length-X/2 columns in place at stride 2Y
 PAIR2D = [((2048, 2048), 1), ((2048, 2048), 2), ((2048, 1024), 2), ((1024, 1024), 3), ((1024, 2048), 1)]
 
 
@@ -381,6 +382,7 @@ def test_pair2d_plans(torch_dev, oracle, monkeypatch, dims, batch, inverse):
 
     from sip_amd.fft import Plan
     monkeypatch.setenv("ASDFFT_PAIR2D", "1")
+    monkeypatch.setenv("ASDFFT_QUAD2D", "0")
     rng = np.random.default_rng(sum(dims) + batch + inverse)
     x = _uniform_c(rng, (batch,) + dims)
     p = Plan("c2c", dims, batch, inverse)
@@ -391,9 +393,8 @@ def test_pair2d_plans(torch_dev, oracle, monkeypatch, dims, batch, inverse):
         assert np.array_equal(xt.cpu().numpy(), x)  # the input is left as it was
     finally:
         p.close()
-    last = batch - 1
-    want = oracle.c2c_nd(x[last:], 2, inverse)
-    err = per_row_rel(got[last:], want, 2).max()
+    want = oracle.c2c_nd(x, 2, inverse)  # every slice
+    err = per_row_rel(got, want, 2).max()
     assert err <= tol_for(dims), (dims, batch, inverse, err)
 
 
@@ -445,3 +446,37 @@ def test_large_real_c2r(torch_dev, oracle, n, batch, inverse):
     assert got.shape == want.shape
     err = per_row_rel(got, want, 1).max()
     assert err <= tol_for((n,)), (err, tol_for((n,)))
+
+
+# 2D plans whose row pass takes the columns' first radix-4 stage (rows in fours, quarter-length
+# columns in place on the output) where instances exist, else the paired or plain plans
+QUAD_2D = [((2048, 2048), 1, True), ((1024, 1024), 3, False), ((2048, 1024), 2, True), ((1024, 2048), 2, False),
+           ((1024, 512), 4, True), ((64, 256), 5, False)]
+
+
+@pytest.mark.parametrize("dims,batch,inverse", QUAD_2D, ids=str)
+def test_quad_2d_plans(torch_dev, oracle, monkeypatch, dims, batch, inverse):
+    import torch
+
+    from sip_amd.fft import Plan
+    rng = np.random.default_rng(sum(dims) + batch)
+    x = _uniform_c(rng, (batch,) + dims)
+    want = oracle.c2c_nd(x, 2, inverse)
+    outs = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("ASDFFT_QUAD2D", mode)
+        p = Plan("c2c", dims, batch, inverse)
+        monkeypatch.delenv("ASDFFT_QUAD2D")
+        try:
+            quad = "variant=23" in p.describe()
+            if mode == "0":
+                assert not quad
+            elif dims[0] in (1024, 2048) and dims[1] in (1024, 2048):
+                assert quad, p.describe()
+            got = p.execute(torch.from_numpy(x).to(torch_dev)).cpu().numpy()
+        finally:
+            p.close()
+        err = per_row_rel(got, want, 2)
+        assert err.max() <= tol_for(dims), (mode, dims, batch, inverse, err)
+        outs.append(got)
+    assert per_row_rel(outs[0], outs[1].astype(np.complex128), 2).max() <= 1e-6

@@ -74,6 +74,14 @@ def test_multidim_plans_have_no_transposes(monkeypatch):
     p = Plan("c2c", (2048, 2048), 4)
     s = steps(p.describe())
     assert len(s) == 2 and "[in->out]" in s[0] and "[out->out]" in s[1] and p.workspace_size == 0
+    # rows in fours (variant 23: rows n + q 512 with the columns' first radix-4 stage), then
+    # 512-point columns on one LDS plane (variant 15), in place on the output
+    assert field(s[0], "variant") == "23"
+    assert field(s[1], "variant") == "15" and field(s[1], "n") == "512" and field(s[1], "lds") == "34816"
+    p.close()
+    monkeypatch.setenv("ASDFFT_QUAD2D", "0")
+    p = Plan("c2c", (2048, 2048), 4)
+    s = steps(p.describe())
     # paired rows (variant 21: rows n and n + 1024 with the columns' first radix-2 stage), then
     # 1024-point columns on one LDS plane (variant 15), both in place on the output
     assert field(s[0], "variant") == "21"
@@ -91,6 +99,7 @@ def test_multidim_plans_have_no_transposes(monkeypatch):
     s1 = steps(p.describe())[1]
     assert field(s1, "variant") == "15" and field(s1, "tpb") == "16" and field(s1, "lds") == "69632"
     p.close()
+    monkeypatch.delenv("ASDFFT_QUAD2D")
     p = Plan("c2c", (512, 512), 4)  # two planes of 16 columns fit twice in a CU: the ordinary column kernel
     assert field(steps(p.describe())[1], "variant") == "1"
     p.close()
@@ -227,3 +236,22 @@ def test_in_place_steps_read_and_write_the_same_elements():
             assert st.group(1) == st.group(2) and st.group(3) == st.group(4), (kind, dims, batch, line)
         p.close()
     assert checked >= 4
+
+
+def test_2d_rows_in_fours(monkeypatch):
+    """2D C2C with both lengths in {1024, 2048}: the row pass takes four rows per workgroup and
+    the columns' first radix-4 stage (variant 23), the quarter-length columns run in place on
+    the one-plane kernel (variant 15) at stride 4Y.  ASDFFT_QUAD2D=0 falls back to the pairs."""
+    for (X, Y), b in (((2048, 2048), 4), ((1024, 1024), 16), ((2048, 1024), 2)):
+        p = Plan("c2c", (X, Y), b)
+        s = steps(p.describe())
+        p.close()
+        assert len(s) == 2 and field(s[0], "variant") == "23" and field(s[0], "tpb") == "4", s
+        assert field(s[1], "variant") == "15" and field(s[1], "n") == str(X // 4), s
+        assert f"stride={4 * Y}/{4 * Y}" in s[1], s
+        monkeypatch.setenv("ASDFFT_QUAD2D", "0")
+        p = Plan("c2c", (X, Y), b)
+        s = steps(p.describe())
+        p.close()
+        monkeypatch.delenv("ASDFFT_QUAD2D")
+        assert field(s[0], "variant") in ("0", "21"), s
