@@ -286,13 +286,21 @@ bool cols_config(int64_t n, int64_t count, int variant, Config* c, bool prefer_s
     // kernel (8 columns).  ASDFFT_SPLIT_EMAX=0 disables it; ASDFFT_SPLIT_EMAX / ASDFFT_SPLIT_TPB
     // force the register budget / columns per workgroup of a specialised instance (measurement).
     const char* ns = getenv("ASDFFT_NO_STATIC");
+    // ASDFFT_COLS_SPLIT=1: the one-plane kernel for every column pass that has an instance
+    // (measurement)
+    const char* cs = getenv("ASDFFT_COLS_SPLIT");
+    if (cs != nullptr && cs[0] == '1') prefer_split = true;
     const char* se = getenv("ASDFFT_SPLIT_EMAX");
     const int split_only = se ? atoi(se) : -1;
     const char* st = getenv("ASDFFT_SPLIT_TPB");
     const int split_tpb_forced = st ? atoi(st) : 0;
     const int split_variant = split_variant_of(variant);
-    const size_t two_per_cu = kLdsLimit / 2;
-    if (split_variant >= 0 && found && (c->lds > two_per_cu || prefer_split) && split_only != 0 &&
+    // the one-plane kernel wherever the two planes leave fewer than four workgroups per CU and an
+    // instance exists: 512-point columns (512^2 x 64: 100.4 against 122.7 us, the column pass 65.4
+    // against 81.2 us), 1024 and longer.  At 256 points (four per CU either way) the two-plane
+    // kernel stays (60.4 against 62.4 us).
+    const size_t four_per_cu = kLdsLimit / 4;
+    if (split_variant >= 0 && found && (c->lds > four_per_cu || prefer_split) && split_only != 0 &&
         !(ns != nullptr && ns[0] == '1')) {
         bool done = false;
         for (int tpb : {16, 8}) {

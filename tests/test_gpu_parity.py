@@ -321,7 +321,8 @@ def test_twiddle_preload_rows_match(torch_dev, oracle, monkeypatch, n, batch):
 LONG_COLUMNS = [("c2c", (2048, 64), 1, False), ("c2c", (2048, 40), 2, True), ("c2c", (2048, 4, 8), 1, False),
                 ("c2c", (2048, 24), 1, True), ("c2c", (1000, 40), 2, False), ("c2c", (1000, 1000), 1, True),
                 ("c2c", (1024, 40), 1, False), ("c2c", (1024, 1024), 1, True), ("c2c", (4096, 40), 1, False),
-                ("c2c", (4096, 12), 1, True)]
+                ("c2c", (4096, 12), 1, True), ("c2c", (512, 512), 3, False), ("c2c", (512, 96), 3, True),
+                ("c2c", (512, 64, 32), 2, False), ("c2c", (512, 40), 1, True)]
 
 
 @pytest.mark.parametrize("kind,dims,batch,inverse", LONG_COLUMNS, ids=str)

@@ -94,13 +94,19 @@ def test_multidim_plans_have_no_transposes(monkeypatch):
     # 16 columns per workgroup (128-byte runs) in 136 KiB
     assert field(s[1], "variant") == "15" and field(s[1], "tpb") == "16" and field(s[1], "lds") == "139264"
     p.close()
-    monkeypatch.delenv("ASDFFT_PAIR2D")
     p = Plan("c2c", (1024, 1024), 4)  # 1024-point columns: 16 per workgroup on one plane, two per CU
     s1 = steps(p.describe())[1]
     assert field(s1, "variant") == "15" and field(s1, "tpb") == "16" and field(s1, "lds") == "69632"
     p.close()
+    monkeypatch.delenv("ASDFFT_PAIR2D")
     monkeypatch.delenv("ASDFFT_QUAD2D")
-    p = Plan("c2c", (512, 512), 4)  # two planes of 16 columns fit twice in a CU: the ordinary column kernel
+    # 512-point columns: two planes of 16 columns fit only twice in a CU, so the one-plane kernel
+    # (four per CU) takes them; 256-point columns fit four times either way and keep two planes
+    p = Plan("c2c", (512, 512), 4)
+    s1 = steps(p.describe())[1]
+    assert field(s1, "variant") == "15" and field(s1, "lds") == "34816", s1
+    p.close()
+    p = Plan("c2c", (256, 256), 4)
     assert field(steps(p.describe())[1], "variant") == "1"
     p.close()
     p = Plan("c2c", (256, 256, 256), 1)
