@@ -38,6 +38,7 @@ enum Variant : int {
     kVarRowsPair2 = 21,         // 2D rows in pairs n, n + X/2 with the columns' first radix-2 stage (specialised only)
     kVarRowsC2CTwPre = 22,      // rows C2C with every pass's twiddles loaded at kernel start (specialised only)
     kVarRowsQuad4 = 23,         // 2D rows in fours n + q X/4 with the columns' first radix-4 stage (specialised only)
+    kVarRowsQuad4Split = 24,    // kVarRowsQuad4 exchanging through one LDS plane (specialised only)
 };
 
 // Dynamic LDS limit set on every kernel that needs more than 64 KiB: the whole LDS of a

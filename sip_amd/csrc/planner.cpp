@@ -219,6 +219,7 @@ int split_variant_of(int variant)
         case kVarColsC2C: return kVarColsC2CSplit;
         case kVarColsShiftOut: return kVarColsShiftOutSplit;
         case kVarRowsC2C: return kVarRowsC2CSplit;
+        case kVarRowsQuad4: return kVarRowsQuad4Split;
         default: return -1;
     }
 }
@@ -394,6 +395,15 @@ bool stockham_step(int64_t n, int variant, bool cols_mapping, const StepIo& io, 
     if (pair || quad) {
         c.split = false;
         c.lds = (size_t)(8 * c.tpb * c.tstride);
+    }
+    // rows in fours through one LDS plane where an instance exists (ASDFFT_QUAD_SPLIT=0: two planes)
+    const char* qs = getenv("ASDFFT_QUAD_SPLIT");
+    const char* nst = getenv("ASDFFT_NO_STATIC");
+    if (quad && !(qs != nullptr && qs[0] == '0') && !(nst != nullptr && nst[0] == '1') &&
+        find_static(kVarRowsQuad4Split, c.emax, (int)n, c.threads, c.tpb, c.radices.data(), (int)c.radices.size()) >=
+            0) {
+        c.split = true;
+        c.lds = (size_t)(4 * c.tpb * c.tstride);
     }
     std::vector<int> offsets;
     std::vector<float> tab = stockham_table((int)n, c.radices, &offsets);

@@ -921,7 +921,7 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
 // after that).
 template <int EMAX, int MAP, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
 __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, float* lp, int tt, int tid, bool live,
-                                             const Io& io, float2 (&v)[EMAX])
+                                             const Io& io, float2 (&v)[EMAX], int64_t blk)
 {
     constexpr int B = EMAX / R;
     constexpr int NR = N / R;
@@ -958,7 +958,65 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
             Dft<R>::run(&v[b * R]);
         }
     }
-    if constexpr (sizeof...(Rest) == 0) {
+    if constexpr (sizeof...(Rest) == 0 && EPI == kEpiQuadRadix4) {
+        // rows in fours (kEpiQuadRadix4 through one plane): the four rows' outputs go through
+        // the plane once per component, read back so that each thread holds element i of all
+        // four rows for i = threadIdx.x + m 4T; then the columns' first radix-4 stage in
+        // registers, as in lds_epilogue
+        static_assert(MAP == kMapRows, "rows in fours use the rows mapping");
+        constexpr int kM = (N + 4 * T - 1) / (4 * T);
+        float2 c[4 * kM];  // c[4 m + q]: element threadIdx.x + m 4T of row q
+        static_for<0, 2>([&](auto PL) __attribute__((always_inline)) {
+            constexpr bool kIm = decltype(PL)::value == 1;
+            __syncthreads();  // the previous plane's reads are done before it is overwritten
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = tid + b * T;
+                if (j < NR) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) lp_t[pad16(j + r * NR)] = kIm ? v[b * R + r].y : v[b * R + r].x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < kM; ++m) {
+                const int i = (int)threadIdx.x + m * 4 * T;
+                if (i < N) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float x = lp[q * kTstride + pad16(i)];
+                        if constexpr (kIm) {
+                            c[4 * m + q].y = x;
+                        } else {
+                            c[4 * m + q].x = x;
+                        }
+                    }
+                }
+            }
+        });
+        if (blk * 4 + 3 < a.count) {
+            const int64_t nn = blk % a.o_split;
+            const float2 w1 = ldg2(a.tw_aux, nn), w2 = ldg2(a.tw_aux, 2 * nn), w3 = ldg2(a.tw_aux, 3 * nn);
+            const int64_t ob = nn * a.out_outer + (blk / a.o_split) * a.out_outer2;
+            float2* o0 = reinterpret_cast<float2*>(a.out) + ob;
+            float2* o1 = o0 + a.out_inner;
+            float2* o2 = o1 + a.out_inner;
+            float2* o3 = o2 + a.out_inner;
+#pragma unroll
+            for (int m = 0; m < kM; ++m) {
+                const int i = (int)threadIdx.x + m * 4 * T;
+                if (i < N) {
+                    const float2 u0 = c[4 * m], u1 = c[4 * m + 1], u2 = c[4 * m + 2], u3 = c[4 * m + 3];
+                    const float2 a0 = cadd(u0, u2), a1 = csub(u0, u2);
+                    const float2 b0 = cadd(u1, u3), b1 = mul_mi(csub(u1, u3));
+                    o0[i] = conj_if(cadd(a0, b0), io.cout);
+                    o1[i] = conj_if(cmul(cadd(a1, b1), w1), io.cout);
+                    o2[i] = conj_if(cmul(csub(a0, b0), w2), io.cout);
+                    o3[i] = conj_if(cmul(csub(a1, b1), w3), io.cout);
+                }
+            }
+        }
+    } else if constexpr (sizeof...(Rest) == 0) {
         if (io.cout) {
 #pragma unroll
             for (int e = 0; e < B * R; ++e) v[e].y = -v[e].y;
@@ -1023,7 +1081,8 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
             }
         });
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
-        split_passes<EMAX, MAP, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lp, tt, tid, live, io, v);
+        split_passes<EMAX, MAP, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lp, tt, tid, live, io, v,
+                                                                                     blk);
     }
 }
 
@@ -1040,7 +1099,7 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
     Io io;
     thread_setup<MAP, kProComplex, EPI>(a, g, tt, tid, blk, live, io);
     float2 v[EMAX];
-    split_passes<EMAX, MAP, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, smem, tt, tid, live, io, v);
+    split_passes<EMAX, MAP, EPI, N, T, TPB, 0, 1, 0, Rs...>(a, g, smem, tt, tid, live, io, v, blk);
 }
 
 // ------------------------------------------------------------------ launch helper
@@ -1095,6 +1154,8 @@ hipError_t launch_static(const LaunchShape& s, const FftArgs& args, hipStream_t 
         return launch_kernel<stockham_split_kernel<EMAX, kMapCols, kEpiShift, N, T, TPB, Rs...>>(s, args, stream);
     } else if constexpr (V == kVarRowsC2CSplit) {
         return launch_kernel<stockham_split_kernel<EMAX, kMapRows, kEpiComplex, N, T, TPB, Rs...>>(s, args, stream);
+    } else if constexpr (V == kVarRowsQuad4Split) {
+        return launch_kernel<stockham_split_kernel<EMAX, kMapRows, kEpiQuadRadix4, N, T, TPB, Rs...>>(s, args, stream);
     } else if constexpr (V == kVarRowsC2CTwPre) {
         return launch_kernel<stockham_static_pre_kernel<EMAX, kMapRows, kProComplex, kEpiComplex, N, T, TPB, Rs...>>(
             s, args, stream);

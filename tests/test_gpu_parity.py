@@ -464,20 +464,25 @@ def test_quad_2d_plans(torch_dev, oracle, monkeypatch, dims, batch, inverse):
     x = _uniform_c(rng, (batch,) + dims)
     want = oracle.c2c_nd(x, 2, inverse)
     outs = []
-    for mode in ("1", "0"):
+    # rows in fours through one LDS plane (variant 24, the default), through two (variant 23),
+    # and the paired / plain plans
+    for mode, split, expect in (("1", "1", "variant=24"), ("1", "0", "variant=23"), ("0", "1", None)):
         monkeypatch.setenv("ASDFFT_QUAD2D", mode)
+        monkeypatch.setenv("ASDFFT_QUAD_SPLIT", split)
         p = Plan("c2c", dims, batch, inverse)
         monkeypatch.delenv("ASDFFT_QUAD2D")
+        monkeypatch.delenv("ASDFFT_QUAD_SPLIT")
         try:
-            quad = "variant=23" in p.describe()
-            if mode == "0":
+            quad = "variant=23" in p.describe() or "variant=24" in p.describe()
+            if expect is None:
                 assert not quad
             elif dims[0] in (1024, 2048) and dims[1] in (1024, 2048):
-                assert quad, p.describe()
+                assert expect in p.describe(), p.describe()
             got = p.execute(torch.from_numpy(x).to(torch_dev)).cpu().numpy()
         finally:
             p.close()
         err = per_row_rel(got, want, 2)
-        assert err.max() <= tol_for(dims), (mode, dims, batch, inverse, err)
+        assert err.max() <= tol_for(dims), (mode, split, dims, batch, inverse, err)
         outs.append(got)
-    assert per_row_rel(outs[0], outs[1].astype(np.complex128), 2).max() <= 1e-6
+    for other in outs[1:]:
+        assert per_row_rel(outs[0], other.astype(np.complex128), 2).max() <= 1e-6

@@ -74,9 +74,9 @@ def test_multidim_plans_have_no_transposes(monkeypatch):
     p = Plan("c2c", (2048, 2048), 4)
     s = steps(p.describe())
     assert len(s) == 2 and "[in->out]" in s[0] and "[out->out]" in s[1] and p.workspace_size == 0
-    # rows in fours (variant 23: rows n + q 512 with the columns' first radix-4 stage), then
-    # 512-point columns on one LDS plane (variant 15), in place on the output
-    assert field(s[0], "variant") == "23"
+    # rows in fours through one LDS plane (variant 24: rows n + q 512 with the columns' first
+    # radix-4 stage), then 512-point columns on one LDS plane (variant 15), in place on the output
+    assert field(s[0], "variant") == "24"
     assert field(s[1], "variant") == "15" and field(s[1], "n") == "512" and field(s[1], "lds") == "34816"
     p.close()
     monkeypatch.setenv("ASDFFT_QUAD2D", "0")
@@ -252,7 +252,7 @@ def test_2d_rows_in_fours(monkeypatch):
         p = Plan("c2c", (X, Y), b)
         s = steps(p.describe())
         p.close()
-        assert len(s) == 2 and field(s[0], "variant") == "23" and field(s[0], "tpb") == "4", s
+        assert len(s) == 2 and field(s[0], "variant") == "24" and field(s[0], "tpb") == "4", s
         assert field(s[1], "variant") == "15" and field(s[1], "n") == str(X // 4), s
         assert f"stride={4 * Y}/{4 * Y}" in s[1], s
         monkeypatch.setenv("ASDFFT_QUAD2D", "0")
