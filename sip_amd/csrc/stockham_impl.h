@@ -545,6 +545,22 @@ __device__ __forceinline__ int64_t logical_block(int remap)
     return (int64_t)x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// q = x / d, r = x % d for x >= 0, d > 0.  Operands that fit 32 bits (every launch in practice)
+// take the 32-bit division, a few instructions instead of the 64-bit expansion, which sits in
+// front of every thread's first load.
+__device__ __forceinline__ void divmod_nonneg(int64_t x, int64_t d, int64_t& q, int64_t& r)
+{
+    if ((((uint64_t)x | (uint64_t)d) >> 32) == 0) {
+        const uint32_t x32 = (uint32_t)x, d32 = (uint32_t)d;
+        const uint32_t q32 = x32 / d32;
+        q = q32;
+        r = x32 - q32 * d32;
+    } else {
+        q = x / d;
+        r = x - q * d;
+    }
+}
+
 // ------------------------------------------------------------------ shared kernel body
 template <int MAP, int PRO, int EPI>
 __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int& tt, int& tid, int64_t& blk, bool& live,
@@ -571,8 +587,8 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
     io.in_stride = a.in_stride;
     io.out_stride = a.out_stride;
     if (live) {
-        io.col = t % a.cols;
-        const int64_t o = t / a.cols;
+        int64_t o = t;  // outer index; one transform per outer step when cols == 1 (uniform)
+        if (a.cols != 1) divmod_nonneg(t, a.cols, o, io.col);
         int64_t oi = o;     // outer index read (input shift: rolled within groups of in_o_mod)
         int64_t co = io.col;  // column index stored (output shift: rolled)
         if constexpr (PRO == kProShift) {
@@ -590,8 +606,11 @@ __device__ __forceinline__ void thread_setup(const FftArgs& a, const Geo& g, int
         int64_t ib = io.col * a.in_inner + oi * a.in_outer;
         int64_t ob = co * a.out_inner + o * a.out_outer;
         if (a.o_split > 0) {  // uniform
-            ib = io.col * a.in_inner + (oi % a.o_split) * a.in_outer + (oi / a.o_split) * a.in_outer2;
-            ob = co * a.out_inner + (o % a.o_split) * a.out_outer + (o / a.o_split) * a.out_outer2;
+            int64_t iq, ir, oq, orr;
+            divmod_nonneg(oi, a.o_split, iq, ir);
+            divmod_nonneg(o, a.o_split, oq, orr);
+            ib = io.col * a.in_inner + ir * a.in_outer + iq * a.in_outer2;
+            ob = co * a.out_inner + orr * a.out_outer + oq * a.out_outer2;
         }
         if constexpr (PRO == kProReal) {
             io.in = reinterpret_cast<const float*>(a.in) + ib;
