@@ -1080,11 +1080,8 @@ bool quad2d(int64_t X, int64_t Y, int64_t B, bool inv, BuiltPlan* out)
     ri.out_outer = 4 * Y;
     ri.out_outer2 = X * Y;
     ri.o_split = q;
-    if (!stockham_step(Y, kVarRowsQuad4, false, ri, inv, 0, 4, &rows, &why) ||
-        (rows.shape.static_id < 0 && quad2d_mode() != 1)) {
-        return false;
-    }
-    if (rows.shape.static_id < 0) return false;  // the four-row workgroup exists only as an instance
+    // the four-row workgroup (more threads than the run-time kernel's block) exists only as an instance
+    if (!stockham_step(Y, kVarRowsQuad4, false, ri, inv, 0, 4, &rows, &why) || rows.shape.static_id < 0) return false;
     rows.args.tw_aux = pt ? pt->dev : nullptr;
     rows.tables_ready = rows.tables_ready && pt != nullptr;
     if (pt) rows.tables.push_back(pt);
