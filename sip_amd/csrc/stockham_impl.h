@@ -383,6 +383,15 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("" ::: "memory");
 }
 
+// Ordering point of a wavefront's own LDS traffic (see WS below): wait for its outstanding LDS
+// accesses and keep the compiler from moving LDS accesses across.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // ------------------------------------------------------------------ one pass
 // RT: runtime geometry (the pass loop dispatches on the radix at run time).  The
 // opaque copies stop the compiler from hoisting every radix case's address
@@ -391,7 +400,12 @@ __device__ __forceinline__ void lds_barrier()
 // workgroup's stores of the previous tile keep draining while the next tile runs.
 // PRE: this pass's twiddles were loaded into registers at kernel start (twr[b * (R - 1) + r - 1]
 // for butterfly b of the thread), so no table load waits inside the pass.
-template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false>
+// WS: every transform lies inside one wavefront (rows, T divides 64) and nothing crosses
+// transforms through LDS: the exchange needs no workgroup barrier.  A wavefront's LDS accesses
+// complete in issue order, so its reads of a pass's inputs are done before its writes of the
+// outputs land, and its writes before the next pass's reads; only the compiler has to be kept
+// from reordering them (and the waves of a workgroup no longer wait for one another).
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false, bool WS = false>
 __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
                                               bool first, bool last, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io, const float2* twr = nullptr)
@@ -473,8 +487,11 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     }
 
     // 3. scatter
+    static_assert(!WS || (MAP == kMapRows && !kEpiViaLds), "wave-local exchange: rows, no cross-transform epilogue");
     if (!last || kEpiViaLds) {
-        if constexpr (LB) {
+        if constexpr (WS) {
+            wave_lds_sync();
+        } else if constexpr (LB) {
             lds_barrier();
         } else {
             __syncthreads();
@@ -495,7 +512,9 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 }
             }
         }
-        if constexpr (LB) {
+        if constexpr (WS) {
+            wave_lds_sync();
+        } else if constexpr (LB) {
             lds_barrier();
         } else {
             __syncthreads();
@@ -813,12 +832,20 @@ __global__ void __launch_bounds__(max_block<EMAX>()) stockham_kernel(const FftAr
 // ------------------------------------------------------------------ specialised kernel
 // Same passes with N, threads, transforms per workgroup and the radix sequence as
 // template constants (instances listed in static_instances.h).
+// Row transforms that fit inside one wavefront with an epilogue that stays inside the transform.
+template <int MAP, int EPI, int T>
+constexpr bool wave_local_exchange()
+{
+    return MAP == kMapRows && T <= 64 && 64 % T == 0 &&
+           (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiReal || EPI == kEpiPlanar || EPI == kEpiShift);
+}
+
 template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
 __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io)
 {
-    stockham_pass<EMAX, R, MAP, PRO, EPI, false>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt,
-                                                 tid, live, io);
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, false, wave_local_exchange<MAP, EPI, T>()>(
+        a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt, tid, live, io);
     if constexpr (sizeof...(Rest) > 0) {
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
         static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lre, lim, tt, tid,
