@@ -405,7 +405,10 @@ __device__ __forceinline__ void wave_lds_sync()
 // complete in issue order, so its reads of a pass's inputs are done before its writes of the
 // outputs land, and its writes before the next pass's reads; only the compiler has to be kept
 // from reordering them (and the waves of a workgroup no longer wait for one another).
-template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false, bool WS = false>
+// IL: the exchange keeps each point as one float2 (real, imaginary interleaved) in a single LDS
+// array over both planes' space, one 64-bit access per point instead of two 32-bit ones.
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false, bool WS = false,
+          bool IL = false>
 __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
                                               bool first, bool last, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io, const float2* twr = nullptr)
@@ -426,6 +429,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     const int toff = (MAP == kMapRows) ? tt * g.tstride : tt;
     float* lre_t = lre + toff;
     float* lim_t = lim + toff;
+    float2* lc_t = reinterpret_cast<float2*>(lre) + toff;  // IL: one float2 per point
     auto kmod = [&](int j) __attribute__((always_inline)) {
         if constexpr (RT) {
             return j - (int)__umulhi((uint32_t)j, magic) * span;
@@ -459,7 +463,11 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int at = lds_rel<MAP, RT>(g, j, pj, r * nr, false);
-                    v[b * R + r] = make_float2(lre_t[at], lim_t[at]);
+                    if constexpr (IL) {
+                        v[b * R + r] = lc_t[at];
+                    } else {
+                        v[b * R + r] = make_float2(lre_t[at], lim_t[at]);
+                    }
                 }
             }
         }
@@ -507,8 +515,12 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int at = lds_rel<MAP, RT>(g, base, pb, r * span, base16);
-                    lre_t[at] = v[b * R + r].x;
-                    lim_t[at] = v[b * R + r].y;
+                    if constexpr (IL) {
+                        lc_t[at] = v[b * R + r];
+                    } else {
+                        lre_t[at] = v[b * R + r].x;
+                        lim_t[at] = v[b * R + r].y;
+                    }
                 }
             }
         }
@@ -840,12 +852,26 @@ constexpr bool wave_local_exchange()
            (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiReal || EPI == kEpiPlanar || EPI == kEpiShift);
 }
 
+// Row kernels whose epilogue does not read the exchange buffer (it may then hold interleaved
+// points).  ASDFFT_LDS_PLANES builds (-D) keep the two-plane exchange for measurement.
+template <int MAP, int EPI>
+constexpr bool interleaved_exchange()
+{
+#ifdef ASDFFT_LDS_PLANES
+    return false;
+#else
+    return MAP == kMapRows &&
+           (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiReal || EPI == kEpiPlanar || EPI == kEpiShift);
+#endif
+}
+
 template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
 __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io)
 {
-    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, false, wave_local_exchange<MAP, EPI, T>()>(
-        a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt, tid, live, io);
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, false, wave_local_exchange<MAP, EPI, T>(),
+                  interleaved_exchange<MAP, EPI>()>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt,
+                                                    tid, live, io);
     if constexpr (sizeof...(Rest) > 0) {
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
         static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lre, lim, tt, tid,
@@ -925,8 +951,8 @@ template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int
 __device__ __forceinline__ void static_passes_pre(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt,
                                                   int tid, bool live, const Io& io, const float2* twr)
 {
-    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, true>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0,
-                                                             lre, lim, tt, tid, live, io, twr);
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, true, false, interleaved_exchange<MAP, EPI>()>(
+        a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt, tid, live, io, twr);
     if constexpr (sizeof...(Rest) > 0) {
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
         static_passes_pre<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(
