@@ -392,6 +392,15 @@ __device__ __forceinline__ void wave_lds_sync()
     asm volatile("" ::: "memory");
 }
 
+// Kernels whose epilogue does not read the exchange buffer (it may then hold interleaved points,
+// IL below); the epilogues that go through LDS read the two planes.
+template <int MAP, int EPI>
+constexpr bool interleaved_exchange()
+{
+    return !(EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar || EPI == kEpiRowsOut ||
+             EPI == kEpiPairRadix2 || EPI == kEpiQuadRadix4);
+}
+
 // ------------------------------------------------------------------ one pass
 // RT: runtime geometry (the pass loop dispatches on the radix at run time).  The
 // opaque copies stop the compiler from hoisting every radix case's address
@@ -557,8 +566,8 @@ __device__ __forceinline__ void maybe_pass(const FftArgs& a, const Geo& g, int p
         const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
         const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
         const int tw_offset = __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
-        stockham_pass<EMAX, R, MAP, PRO, EPI, true, LB>(a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre,
-                                                        lim, tt, tid, live, io);
+        stockham_pass<EMAX, R, MAP, PRO, EPI, true, LB, false, false, interleaved_exchange<MAP, EPI>()>(
+            a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre, lim, tt, tid, live, io);
     }
     // radices above EMAX are never scheduled: the planner checks radix <= emax before launch
 }
@@ -852,18 +861,6 @@ constexpr bool wave_local_exchange()
            (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiReal || EPI == kEpiPlanar || EPI == kEpiShift);
 }
 
-// Row kernels whose epilogue does not read the exchange buffer (it may then hold interleaved
-// points).  ASDFFT_LDS_PLANES builds (-D) keep the two-plane exchange for measurement.
-template <int MAP, int EPI>
-constexpr bool interleaved_exchange()
-{
-#ifdef ASDFFT_LDS_PLANES
-    return false;
-#else
-    return MAP == kMapRows &&
-           (EPI == kEpiComplex || EPI == kEpiHalf || EPI == kEpiReal || EPI == kEpiPlanar || EPI == kEpiShift);
-#endif
-}
 
 template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int SPAN, int TWOFF, int R, int... Rest>
 __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
