@@ -6,7 +6,8 @@
 // reading its inputs at stride n/R and writing them to the autosort position
 // (j - k)*R + k + r*L (k = j mod L).  The first pass reads HBM directly (lane
 // -contiguous float2 streams), the last pass writes HBM directly; the passes in
-// between exchange through LDS (separate re/im planes, one pad word per 16).
+// between exchange through LDS (interleaved float2 points, one pad point per 16; the one-plane
+// kernels below exchange real and imaginary parts in turn).
 // Inverse transforms use conj(DFT(conj(x))): only forward twiddles exist.
 //
 // Capability it replaces: the reference's per-row Stockham device kernels
@@ -392,13 +393,11 @@ __device__ __forceinline__ void wave_lds_sync()
     asm volatile("" ::: "memory");
 }
 
-// Kernels whose epilogue does not read the exchange buffer (it may then hold interleaved points,
-// IL below); the epilogues that go through LDS read the two planes.
-template <int MAP, int EPI>
-constexpr bool interleaved_exchange()
+// Point `at` of the exchange buffer (interleaved float2 over both planes' space); the epilogues
+// that go through LDS read the passes' results with it.
+__device__ __forceinline__ float2 lds_point(const float* lre, int at)
 {
-    return !(EPI == kEpiR2CEven || EPI == kEpiTranspose || EPI == kEpiTransposePlanar || EPI == kEpiRowsOut ||
-             EPI == kEpiPairRadix2 || EPI == kEpiQuadRadix4);
+    return reinterpret_cast<const float2*>(lre)[at];
 }
 
 // ------------------------------------------------------------------ one pass
@@ -414,10 +413,10 @@ constexpr bool interleaved_exchange()
 // complete in issue order, so its reads of a pass's inputs are done before its writes of the
 // outputs land, and its writes before the next pass's reads; only the compiler has to be kept
 // from reordering them (and the waves of a workgroup no longer wait for one another).
-// IL: the exchange keeps each point as one float2 (real, imaginary interleaved) in a single LDS
-// array over both planes' space, one 64-bit access per point instead of two 32-bit ones.
-template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false, bool WS = false,
-          bool IL = false>
+// The exchange keeps each point as one float2 (real, imaginary interleaved) in a single LDS array
+// over both planes' space (lre; lim is its upper half): one 64-bit access per point instead of
+// two 32-bit ones (at 4096 points 48 LDS instructions per wave instead of 128).
+template <int EMAX, int R, int MAP, int PRO, int EPI, bool RT, bool LB = false, bool PRE = false, bool WS = false>
 __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span, uint32_t magic, int tw_offset,
                                               bool first, bool last, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io, const float2* twr = nullptr)
@@ -436,9 +435,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
     const int T = g.T;
     // this transform's LDS block (rows) or column (columns)
     const int toff = (MAP == kMapRows) ? tt * g.tstride : tt;
-    float* lre_t = lre + toff;
-    float* lim_t = lim + toff;
-    float2* lc_t = reinterpret_cast<float2*>(lre) + toff;  // IL: one float2 per point
+    float2* lc_t = reinterpret_cast<float2*>(lre) + toff;  // one float2 per point
     auto kmod = [&](int j) __attribute__((always_inline)) {
         if constexpr (RT) {
             return j - (int)__umulhi((uint32_t)j, magic) * span;
@@ -472,11 +469,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int at = lds_rel<MAP, RT>(g, j, pj, r * nr, false);
-                    if constexpr (IL) {
-                        v[b * R + r] = lc_t[at];
-                    } else {
-                        v[b * R + r] = make_float2(lre_t[at], lim_t[at]);
-                    }
+                    v[b * R + r] = lc_t[at];
                 }
             }
         }
@@ -524,12 +517,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int at = lds_rel<MAP, RT>(g, base, pb, r * span, base16);
-                    if constexpr (IL) {
-                        lc_t[at] = v[b * R + r];
-                    } else {
-                        lre_t[at] = v[b * R + r].x;
-                        lim_t[at] = v[b * R + r].y;
-                    }
+                    lc_t[at] = v[b * R + r];
                 }
             }
         }
@@ -566,8 +554,8 @@ __device__ __forceinline__ void maybe_pass(const FftArgs& a, const Geo& g, int p
         const int span = __builtin_amdgcn_readfirstlane(a.pass[p].span);
         const uint32_t magic = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.pass[p].magic);
         const int tw_offset = __builtin_amdgcn_readfirstlane(a.pass[p].tw_offset);
-        stockham_pass<EMAX, R, MAP, PRO, EPI, true, LB, false, false, interleaved_exchange<MAP, EPI>()>(
-            a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre, lim, tt, tid, live, io);
+        stockham_pass<EMAX, R, MAP, PRO, EPI, true, LB>(a, g, span, magic, tw_offset, p == 0, p == a.passes - 1, lre,
+                                                        lim, tt, tid, live, io);
     }
     // radices above EMAX are never scheduled: the planner checks radix <= emax before launch
 }
@@ -685,8 +673,8 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                 const int km = (k == 0) ? 0 : m - k;
                 const int at0 = lds_at<MAP>(g, tt, k);
                 const int at1 = lds_at<MAP>(g, tt, km);
-                const float2 zk = make_float2(lre[at0], lim[at0]);
-                const float2 zm = make_float2(lre[at1], lim[at1]);
+                const float2 zk = lds_point(lre, at0);
+                const float2 zm = lds_point(lre, at1);
                 const float2 wk = ldg2(a.tw_aux, k);
                 const float2 wm = ldg2(a.tw_aux, m - k);
                 const float2 s0 = make_float2(zk.x + zm.x, zk.y - zm.y);
@@ -716,7 +704,7 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                 float2* ob = reinterpret_cast<float2*>(a.out) + o;
                 for (int i = lane / g.tpb; i < g.n; i += step) {
                     const int at = lds_at<MAP>(g, tt2, i);
-                    ob[i * s] = conj_if(make_float2(lre[at], lim[at]), io.cout);
+                    ob[i * s] = conj_if(lds_point(lre, at), io.cout);
                 }
             } else {
                 float* obr = reinterpret_cast<float*>(a.out) + o;
@@ -724,8 +712,9 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
                 const float sgn = io.cout ? -1.0f : 1.0f;
                 for (int i = lane / g.tpb; i < g.n; i += step) {
                     const int at = lds_at<MAP>(g, tt2, i);
-                    obr[i * s] = lre[at];
-                    obi[i * s] = sgn * lim[at];
+                    const float2 x = lds_point(lre, at);
+                    obr[i * s] = x.x;
+                    obi[i * s] = sgn * x.y;
                 }
             }
         }
@@ -743,8 +732,8 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
             float2* out1 = out0 + a.out_inner;
             for (int i = threadIdx.x; i < g.n; i += g.T * 2) {
                 const int at0 = lds_at<MAP>(g, 0, i), at1 = lds_at<MAP>(g, 1, i);
-                const float2 u = make_float2(lre[at0], lim[at0]);
-                const float2 v = make_float2(lre[at1], lim[at1]);
+                const float2 u = lds_point(lre, at0);
+                const float2 v = lds_point(lre, at1);
                 out0[i] = conj_if(cadd(u, v), io.cout);
                 out1[i] = conj_if(cmul(csub(u, v), w), io.cout);
             }
@@ -766,8 +755,8 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
             for (int i = threadIdx.x; i < g.n; i += g.T * 4) {
                 const int at0 = lds_at<MAP>(g, 0, i), at1 = lds_at<MAP>(g, 1, i);
                 const int at2 = lds_at<MAP>(g, 2, i), at3 = lds_at<MAP>(g, 3, i);
-                const float2 u0 = make_float2(lre[at0], lim[at0]), u1 = make_float2(lre[at1], lim[at1]);
-                const float2 u2 = make_float2(lre[at2], lim[at2]), u3 = make_float2(lre[at3], lim[at3]);
+                const float2 u0 = lds_point(lre, at0), u1 = lds_point(lre, at1);
+                const float2 u2 = lds_point(lre, at2), u3 = lds_point(lre, at3);
                 const float2 a0 = cadd(u0, u2), a1 = csub(u0, u2);
                 const float2 b0 = cadd(u1, u3), b1 = mul_mi(csub(u1, u3));
                 o0[i] = conj_if(cadd(a0, b0), io.cout);
@@ -788,7 +777,7 @@ __device__ __forceinline__ void lds_epilogue(const FftArgs& a, const Geo& g, flo
             if (t2 >= a.count) break;  // e only grows: the rest of this thread's elements are past the end
             const int64_t o = (t2 % a.cols) * a.out_inner + (t2 / a.cols) * a.out_outer;
             const int at = lds_at<MAP>(g, tt2, i);
-            reinterpret_cast<float2*>(a.out)[o + i] = conj_if(make_float2(lre[at], lim[at]), io.cout);
+            reinterpret_cast<float2*>(a.out)[o + i] = conj_if(lds_point(lre, at), io.cout);
         }
     }
 }
@@ -866,9 +855,8 @@ template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int
 __device__ __forceinline__ void static_passes(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt, int tid,
                                               bool live, const Io& io)
 {
-    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, false, wave_local_exchange<MAP, EPI, T>(),
-                  interleaved_exchange<MAP, EPI>()>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt,
-                                                    tid, live, io);
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, false, wave_local_exchange<MAP, EPI, T>()>(
+        a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt, tid, live, io);
     if constexpr (sizeof...(Rest) > 0) {
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
         static_passes<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(a, g, lre, lim, tt, tid,
@@ -948,8 +936,8 @@ template <int EMAX, int MAP, int PRO, int EPI, int N, int T, int TPB, int P, int
 __device__ __forceinline__ void static_passes_pre(const FftArgs& a, const Geo& g, float* lre, float* lim, int tt,
                                                   int tid, bool live, const Io& io, const float2* twr)
 {
-    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, true, false, interleaved_exchange<MAP, EPI>()>(
-        a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0, lre, lim, tt, tid, live, io, twr);
+    stockham_pass<EMAX, R, MAP, PRO, EPI, false, false, true>(a, g, SPAN, 0u, TWOFF, P == 0, sizeof...(Rest) == 0,
+                                                             lre, lim, tt, tid, live, io, twr);
     if constexpr (sizeof...(Rest) > 0) {
         constexpr int kNextOff = TWOFF + (SPAN > 1 ? SPAN * (R - 1) : 0);
         static_passes_pre<EMAX, MAP, PRO, EPI, N, T, TPB, P + 1, SPAN * R, kNextOff, Rest...>(
