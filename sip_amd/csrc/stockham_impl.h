@@ -966,6 +966,23 @@ __global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) sto
 }
 
 // ------------------------------------------------------------------ one-plane exchange
+// Occupancy target of the one-plane row kernels.  A 512-thread EMAX-16 row workgroup (8
+// waves; the 2D rows in fours) takes about 34 KiB of LDS, so four fit a CU by LDS; four need 8
+// waves per SIMD, at most 64 VGPRs.  Left to itself the compiler takes 72 VGPRs there (7 waves
+// per SIMD: three workgroups per CU).  With the target, and each butterfly's twiddles loaded
+// only after the previous butterfly is done (otherwise the compiler spills to scratch to meet
+// it), the 2048-point rows-in-fours pass of 2048^2 x 4 took 59.0-59.3 against 60.9-61.8 us
+// (the whole transform stays at 108-109 us: its two passes' 537 MB of reads and writes in
+// 108 us are already the 5.0 TB/s of a plain device copy).  The 512-thread one-plane column
+// kernels measured slower both ways (512-point columns: 66.1 staged, 67.1 staged at the
+// target, against 64.4 us), and 32 columns per 1024-thread workgroup at the target (56 VGPRs,
+// two workgroups per CU) no faster (65.0), so the column kernels keep the compiler's choice.
+template <int EMAX, int MAP, int T, int TPB>
+constexpr int split_min_waves()
+{
+    return (EMAX <= 16 && MAP == kMapRows && static_block_bound<EMAX, MAP, T, TPB>() == 512) ? 8 : 1;
+}
+
 // Specialised kernel whose passes exchange through ONE LDS plane: first the real parts of
 // every point, then the imaginary parts.  It needs half the LDS of the two-plane exchange,
 // so twice the workgroups fit a CU and one's loads overlap the others' passes; the cost is
@@ -984,6 +1001,7 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
     constexpr int NR = N / R;
     constexpr int PITCH = TPB + 1;  // column layout: slot of element i of column tt = i * PITCH + tt
     constexpr int kTstride = N + (N >> 4) + 1;  // rows layout: one padded block per transform
+    constexpr bool kStaged = split_min_waves<EMAX, MAP, T, TPB>() > 1;
     static_assert(B >= 1, "radix larger than the per-thread register budget");
     float* lp_t = (MAP == kMapRows) ? lp + tt * kTstride : lp + tt;
     if constexpr (P == 0) {
@@ -1013,6 +1031,12 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
                 for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
             }
             Dft<R>::run(&v[b * R]);
+        }
+        if constexpr (SPAN > 1 && kStaged) {
+            // butterfly b is done before the next one's twiddles load: one butterfly's
+            // twiddles in registers at a time keeps the kernel inside its occupancy target
+#pragma unroll
+            for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[b * R + r].x), "+v"(v[b * R + r].y)::"memory");
         }
     }
     if constexpr (sizeof...(Rest) == 0 && EPI == kEpiQuadRadix4) {
@@ -1146,7 +1170,8 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
 // Rows mapping keeps transforms per workgroup at run time (the launch bound is the
 // register-budget block); columns take it as a constant.
 template <int EMAX, int MAP, int EPI, int N, int T, int TPB, int... Rs>
-__global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>())) stockham_split_kernel(const FftArgs a)
+__global__ void __launch_bounds__((static_block_bound<EMAX, MAP, T, TPB>()))
+__attribute__((amdgpu_waves_per_eu(split_min_waves<EMAX, MAP, T, TPB>()))) stockham_split_kernel(const FftArgs a)
 {
     extern __shared__ float smem[];
     const Geo g{N, T, (MAP == kMapRows) ? a.tpb : TPB, N + (N >> 4) + 1};
