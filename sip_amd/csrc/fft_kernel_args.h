@@ -75,6 +75,7 @@ struct FftArgs {
     int conj_in, conj_out;
     int big_shift;          // four-step twiddle split
     int xcd_remap;          // 1: give each XCD a contiguous range of workgroups (shared L2 lines)
+    int stream_in;          // 1: the first pass reads its complex input with non-temporal loads
     int64_t count;          // transforms in this launch
     int64_t cols;
     int64_t in_inner, in_outer, in_stride;

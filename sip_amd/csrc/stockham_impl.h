@@ -305,6 +305,18 @@ __device__ __forceinline__ int64_t stride_of(int64_t s)
     return s;
 }
 
+// Non-temporal load of point i of an interleaved complex input.  A multi-pass plan's first
+// pass reads its input exactly once; loading it non-temporally keeps it from displacing, in the
+// 256 MB device cache, the intermediate that the pass writes and the next pass reads back.
+template <int MAP>
+__device__ __forceinline__ float2 load_streamed(const Io& io, int i)
+{
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v q = __builtin_nontemporal_load(
+        reinterpret_cast<const f2v*>(reinterpret_cast<const float2*>(io.in) + i * stride_of<MAP>(io.in_stride)));
+    return make_float2(q.x, q.y);
+}
+
 template <int MAP, int PRO>
 __device__ __forceinline__ float2 load_input(const FftArgs& a, const Io& io, int n, int i)
 {
@@ -446,13 +458,30 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
 
     // 1. gather
     if (first) {
+        auto gather = [&](auto streamed) __attribute__((always_inline)) {
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const int j = tid + b * T;
-            if (live && j < nr) {
+            for (int b = 0; b < B; ++b) {
+                const int j = tid + b * T;
+                if (live && j < nr) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
+                    for (int r = 0; r < R; ++r) {
+                        if constexpr (decltype(streamed)::value) {
+                            v[b * R + r] = load_streamed<MAP>(io, j + r * nr);
+                        } else {
+                            v[b * R + r] = load_input<MAP, PRO>(a, io, n, j + r * nr);
+                        }
+                    }
+                }
             }
+        };
+        if constexpr (PRO == kProComplex) {
+            if (a.stream_in) {  // uniform
+                gather(IC<1>{});
+            } else {
+                gather(IC<0>{});
+            }
+        } else {
+            gather(IC<0>{});
         }
         if constexpr (PRO == kProComplex || PRO == kProPlanar || PRO == kProShift) {
             if (io.cin) {  // uniform: one scalar branch instead of a XOR per point
@@ -1007,13 +1036,26 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
     if constexpr (P == 0) {
         const float2* in2 = reinterpret_cast<const float2*>(io.in);
         const int64_t s = stride_of<MAP>(io.in_stride);
+        auto gather = [&](auto streamed) __attribute__((always_inline)) {
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const int j = tid + b * T;
-            if (live && j < NR) {
+            for (int b = 0; b < B; ++b) {
+                const int j = tid + b * T;
+                if (live && j < NR) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[b * R + r] = in2[(j + r * NR) * s];
+                    for (int r = 0; r < R; ++r) {
+                        if constexpr (decltype(streamed)::value) {
+                            v[b * R + r] = load_streamed<MAP>(io, j + r * NR);
+                        } else {
+                            v[b * R + r] = in2[(j + r * NR) * s];
+                        }
+                    }
+                }
             }
+        };
+        if (a.stream_in) {  // uniform
+            gather(IC<1>{});
+        } else {
+            gather(IC<0>{});
         }
         if (io.cin) {
 #pragma unroll

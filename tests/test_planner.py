@@ -261,3 +261,25 @@ def test_2d_rows_in_fours(monkeypatch):
         p.close()
         monkeypatch.delenv("ASDFFT_QUAD2D")
         assert field(s[0], "variant") in ("0", "21"), s
+
+
+def test_streamed_input_on_multi_pass_plans(monkeypatch):
+    """The first launch of a plan whose later launch reads back what it wrote (2D, 3D, the
+    four-step, multi-dimensional C2R) loads its input non-temporally; single-launch plans and
+    first passes that do not read interleaved complex points (odd-length R2C rows) do not.
+    ASDFFT_STREAM_IN=0 turns it off."""
+    marked = [("c2c", (2048, 2048), 4), ("c2c", (256, 256, 256), 1), ("c2c", (65536,), 64),
+              ("r2c", (1000, 1000), 2), ("c2r", (64, 64), 2), ("c2c", (1000, 1000), 1)]
+    unmarked = [("c2c", (4096,), 256), ("r2c", (8192,), 16), ("r2c", (1001, 1001), 2), ("c2c", (1, 4096), 3)]
+    for kind, dims, batch in marked + unmarked:
+        p = Plan(kind, dims, batch)
+        s = steps(p.describe())
+        p.close()
+        want = (kind, dims, batch) in marked
+        assert ("streamed-input" in s[0]) == want, (kind, dims, batch, s)
+        assert not any("streamed-input" in line for line in s[1:]), s
+    monkeypatch.setenv("ASDFFT_STREAM_IN", "0")
+    for kind, dims, batch in marked:
+        p = Plan(kind, dims, batch)
+        assert "streamed-input" not in p.describe(), (kind, dims, batch)
+        p.close()
