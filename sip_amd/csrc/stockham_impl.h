@@ -474,7 +474,7 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 }
             }
         };
-        if constexpr (PRO == kProComplex) {
+        if constexpr (PRO == kProComplex && !PRE) {  // the twiddle-preloading kernels run 1D launches only
             if (a.stream_in) {  // uniform
                 gather(IC<1>{});
             } else {
