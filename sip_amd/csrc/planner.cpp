@@ -1445,11 +1445,11 @@ static void mark_streamed_input(BuiltPlan* plan)
     if (plan->steps.size() < 2) return;
     Step& first = plan->steps[0];
     if (first.direct || first.mfma || first.aux || first.wave || first.planar_in || first.src != kBufIn) return;
-    // kernels whose first pass reads interleaved complex points; a column pass only where each
-    // workgroup reads whole 128-byte lines (16 columns or more, rows 128-byte aligned): a
-    // non-temporal load is not kept in L2 for a neighbouring workgroup that reads the rest of a
-    // shared line (ISTFT frames read from a spectrum of 500 frames per row measured 97.7 against
-    // 85.3 us streamed)
+    // kernels with a streamed path: row passes that read interleaved complex points, and the
+    // four-step's first (column) pass where each workgroup reads whole 128-byte lines (16
+    // columns or more, rows 128-byte aligned): a non-temporal load is not kept in L2 for a
+    // neighbouring workgroup that reads the rest of a shared line (ISTFT frames read from a
+    // spectrum of 500 frames per row measured 97.7 against 85.3 us streamed)
     switch (first.shape.variant) {
         case kVarRowsC2C:
         case kVarR2CEven:
@@ -1458,9 +1458,7 @@ static void mark_streamed_input(BuiltPlan* plan)
         case kVarRowsQuad4:
         case kVarRowsQuad4Split:
             break;
-        case kVarColsC2C:
         case kVarColsTwiddle:
-        case kVarColsC2CSplit:
             if (first.args.tpb < 16 || first.args.in_stride % 16 != 0) return;
             break;
         default:

@@ -474,7 +474,10 @@ __device__ __forceinline__ void stockham_pass(const FftArgs& a, Geo g, int span,
                 }
             }
         };
-        if constexpr (PRO == kProComplex && !PRE) {  // the twiddle-preloading kernels run 1D launches only
+        // a streamed path only in kernels that run the first launch of multi-launch plans: row
+        // passes and the four-step's first (column) pass; not the twiddle-preloading kernels
+        // (1D launches only) nor the other column kernels, whose code stays as it was
+        if constexpr (PRO == kProComplex && !PRE && (MAP == kMapRows || EPI == kEpiTwiddle)) {
             if (a.stream_in) {  // uniform
                 gather(IC<1>{});
             } else {
@@ -1052,8 +1055,12 @@ __device__ __forceinline__ void split_passes(const FftArgs& a, const Geo& g, flo
                 }
             }
         };
-        if (a.stream_in) {  // uniform
-            gather(IC<1>{});
+        if constexpr (MAP == kMapRows) {  // the 2D rows in fours run first launches
+            if (a.stream_in) {  // uniform
+                gather(IC<1>{});
+            } else {
+                gather(IC<0>{});
+            }
         } else {
             gather(IC<0>{});
         }

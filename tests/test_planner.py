@@ -265,14 +265,15 @@ def test_2d_rows_in_fours(monkeypatch):
 
 def test_streamed_input_on_multi_pass_plans(monkeypatch):
     """The first launch of a plan whose later launch reads back what it wrote (2D, 3D, the
-    four-step, multi-dimensional C2R) loads its input non-temporally; single-launch plans,
-    first passes that do not read interleaved complex points (odd-length R2C rows), the
-    twiddle-preloading small-launch kernel and column passes whose workgroups share 128-byte
-    lines (C2R 64 x 64: rows of 33 points) do not.  ASDFFT_STREAM_IN=0 turns it off."""
+    four-step) loads its input non-temporally where its kernel has a streamed path (row
+    passes, the four-step's first pass); single-launch plans, first passes that do not read
+    interleaved complex points (odd-length R2C rows), the twiddle-preloading small-launch
+    kernel and first column passes (multi-dimensional C2R) do not.  ASDFFT_STREAM_IN=0 turns it
+    off."""
     marked = [("c2c", (2048, 2048), 4), ("c2c", (256, 256, 256), 1), ("c2c", (65536,), 64),
-              ("r2c", (1000, 1000), 2), ("c2r", (256, 2046), 1), ("c2c", (1000, 1000), 16)]
+              ("r2c", (1000, 1000), 2), ("c2c", (1000, 1000), 16)]
     unmarked = [("c2c", (4096,), 256), ("r2c", (8192,), 16), ("r2c", (1001, 1001), 2), ("c2c", (1, 4096), 3),
-                ("c2r", (64, 64), 2), ("c2c", (1000, 1000), 1)]
+                ("c2r", (64, 64), 2), ("c2r", (256, 2046), 1), ("c2c", (1000, 1000), 1)]
     for kind, dims, batch in marked + unmarked:
         p = Plan(kind, dims, batch)
         s = steps(p.describe())
