@@ -1434,14 +1434,20 @@ static std::string build_plan_steps(const PlanSpec& spec, BuiltPlan* out)
 }
 
 // The first launch of a plan whose later launch reads back what that first launch wrote (2D and
-// 3D transforms, the four-step) loads its input non-temporally: the input is read once, and
-// loads that allocate in the 256 MB device cache would displace the intermediate the next pass
-// reads.  Measured on whole transforms over buffers larger than that cache: 2048^2 x 4 C2C
-// 102.2 against 108.0 us, 1024^2 x 16 91.3 against 97.5 us.  ASDFFT_STREAM_IN=0 turns it off.
+// 3D transforms, the four-step) loads its input non-temporally when that input and the
+// intermediate together are at least the 256 MiB device cache: the input is read once, and loads
+// that allocate in the cache would displace the intermediate the next pass reads.  Measured on
+// whole transforms over rings of buffers larger than the cache: 2048^2 x 4 C2C 105.7 against
+// 111.6 us, 1024^2 x 16 93.0 against 100.0, 4096^2 119.8 against 129.8, 256^3 131.6 against
+// 136.7.  Smaller plans keep ordinary loads: there a call repeated on the same buffers can find
+// all of them in the cache, and streaming gave that up (128^3 x 4 C2C repeated on one buffer
+// pair: 78.3 against 73.2 us), although on a ring the four-step 65536 x 64 gained (34.6 against
+// 37.2 us).  ASDFFT_STREAM_IN=0 / =1 turns it off / on wherever the first kernel has the path.
 static void mark_streamed_input(BuiltPlan* plan)
 {
     const char* e = getenv("ASDFFT_STREAM_IN");
     if (e != nullptr && e[0] == '0') return;
+    const bool forced = (e != nullptr && e[0] == '1');
     if (plan->steps.size() < 2) return;
     Step& first = plan->steps[0];
     if (first.direct || first.mfma || first.aux || first.wave || first.planar_in || first.src != kBufIn) return;
@@ -1467,6 +1473,8 @@ static void mark_streamed_input(BuiltPlan* plan)
     bool read_back = false;
     for (size_t i = 1; i < plan->steps.size(); i++) read_back |= (plan->steps[i].src == first.dst);
     if (!read_back) return;
+    constexpr double kDeviceCacheBytes = 256.0 * 1024 * 1024;
+    if (!forced && first.bytes < kDeviceCacheBytes) return;
     first.args.stream_in = 1;
     first.describe += " streamed-input";
 }

@@ -268,12 +268,14 @@ def test_streamed_input_on_multi_pass_plans(monkeypatch):
     four-step) loads its input non-temporally where its kernel has a streamed path (row
     passes, the four-step's first pass); single-launch plans, first passes that do not read
     interleaved complex points (odd-length R2C rows), the twiddle-preloading small-launch
-    kernel and first column passes (multi-dimensional C2R) do not.  ASDFFT_STREAM_IN=0 turns it
-    off."""
-    marked = [("c2c", (2048, 2048), 4), ("c2c", (256, 256, 256), 1), ("c2c", (65536,), 64),
-              ("r2c", (1000, 1000), 2), ("c2c", (1000, 1000), 16)]
-    unmarked = [("c2c", (4096,), 256), ("r2c", (8192,), 16), ("r2c", (1001, 1001), 2), ("c2c", (1, 4096), 3),
-                ("c2r", (64, 64), 2), ("c2r", (256, 2046), 1), ("c2c", (1000, 1000), 1)]
+    kernel, first column passes (multi-dimensional C2R) and plans whose input and intermediate
+    together are under the 256 MiB device cache (65536 x 64, 128^3 x 4, 1000^2 x 16) do not.
+    ASDFFT_STREAM_IN=0 turns it off, =1 on wherever the first kernel has the path."""
+    marked = [("c2c", (2048, 2048), 4), ("c2c", (256, 256, 256), 1), ("c2c", (65536,), 256),
+              ("r2c", (4096, 4096), 2), ("c2c", (1000, 1000), 20)]
+    unmarked = [("c2c", (4096,), 256), ("r2c", (8192,), 16), ("r2c", (4095, 4095), 2), ("c2c", (1, 4096), 3),
+                ("c2r", (64, 64), 2), ("c2r", (256, 2046), 1), ("c2c", (1000, 1000), 1),
+                ("c2c", (65536,), 64), ("c2c", (128, 128, 128), 4), ("c2c", (1000, 1000), 16)]
     for kind, dims, batch in marked + unmarked:
         p = Plan(kind, dims, batch)
         s = steps(p.describe())
@@ -285,4 +287,9 @@ def test_streamed_input_on_multi_pass_plans(monkeypatch):
     for kind, dims, batch in marked:
         p = Plan(kind, dims, batch)
         assert "streamed-input" not in p.describe(), (kind, dims, batch)
+        p.close()
+    monkeypatch.setenv("ASDFFT_STREAM_IN", "1")
+    for kind, dims, batch in [("c2c", (65536,), 64), ("c2c", (128, 128, 128), 4)]:
+        p = Plan(kind, dims, batch)
+        assert "streamed-input" in steps(p.describe())[0], (kind, dims, batch)
         p.close()
