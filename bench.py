@@ -571,7 +571,9 @@ def bench_serve(torch, dev, rates=(5000, 20000, 50000, 100000, 200000), count=30
         reqs.append(Server.request(kind, dims, batch, False, src.data_ptr(), arena.data_ptr() + off))
         off = (off + ob + 255) // 256 * 256
     torch.cuda.synchronize()
+    # one arrival sequence per rate, replayed by both modes
     rng = np.random.default_rng(12)
+    arrivals_at = {rate: np.cumsum(rng.exponential(1e6 / rate, count)) for rate in rates}
     total_flop = sum(5.0 * math.prod(d) * math.log2(math.prod(d)) * b for _, d, b in work)
     res = {"requests": count, "mix": "85% 1D C2C/R2C N 256..65536 b1..128, 15% 2D C2C <=1024^2",
            "hbm_budget_bytes": 8 << 30, "rates": {}}
@@ -580,7 +582,7 @@ def bench_serve(torch, dev, rates=(5000, 20000, 50000, 100000, 200000), count=30
         try:
             srv.replay(reqs, [0.0] * len(reqs))  # plans and resident classes built before timing
             for rate in rates:
-                arrivals = np.cumsum(rng.exponential(1e6 / rate, count))
+                arrivals = arrivals_at[rate]
                 lat = np.array(srv.replay(reqs, arrivals.tolist()))
                 done = float((arrivals + lat).max())
                 entry = {"p50_us": round(float(np.percentile(lat, 50)), 1),
