@@ -144,7 +144,11 @@ public:
     std::mutex plan_mu;  // plain plans are created by submitters and read by the scheduler
     std::map<std::string, std::unique_ptr<PlainPlan>> plans;
     std::atomic<size_t> workspace_max{0};  // largest workspace of any plan built so far
-    int min_ragged = 2;  // small requests in a batch from which they share one ragged launch
+    // small requests in a batch from which they share one ragged launch (a ragged launch pays a
+    // work-table copy; at 20k req/s two-request ragged batches put the batched p99 about 15 us
+    // behind per-request execs in both runs that replayed the same arrivals; from three on the
+    // two were level within run-to-run noise)
+    int min_ragged = 3;
 
     ~Server();
     bool init(std::string* why);
