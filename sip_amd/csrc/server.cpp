@@ -145,10 +145,8 @@ public:
     std::map<std::string, std::unique_ptr<PlainPlan>> plans;
     std::atomic<size_t> workspace_max{0};  // largest workspace of any plan built so far
     // small requests in a batch from which they share one ragged launch (a ragged launch pays a
-    // work-table copy; at 20k req/s two-request ragged batches put the batched p99 about 15 us
-    // behind per-request execs in both runs that replayed the same arrivals; from three on the
-    // two were level within run-to-run noise)
-    int min_ragged = 3;
+    // work-table copy)
+    int min_ragged = 2;
 
     ~Server();
     bool init(std::string* why);
@@ -486,7 +484,21 @@ void Server::scheduler_loop()
         size_t idle_slots = 0;
         for (const auto& sl : slots) idle_slots += sl->busy ? 0 : 1;
         const size_t share = (pending.size() + std::max<size_t>(idle_slots, 1) - 1) / std::max<size_t>(idle_slots, 1);
-        const size_t limit = std::min<size_t>((size_t)cfg.max_batch, std::max<size_t>(share, 1));
+        size_t limit = std::min<size_t>((size_t)cfg.max_batch, std::max<size_t>(share, 1));
+        if (cfg.batching && limit > 1) {
+            // several requests share a slot only when more requests wait than there are slots
+            // (the device is saturated) and they share a ragged launch: otherwise fewer than
+            // min_ragged small requests would run back to back on this slot's stream, or a short
+            // queue would wait for one ragged launch, while a slot that frees up meanwhile could
+            // run the next request beside it
+            size_t small_head = 0;
+            for (auto it = pending.begin(); it != pending.end() && small_head < limit; ++it) {
+                const Req& h = reqs[*it];
+                if (!(h.ragged && h.bytes <= kRaggedMaxRequestBytes)) break;
+                small_head++;
+            }
+            if (small_head < (size_t)min_ragged || pending.size() <= slots.size()) limit = 1;
+        }
         while (!pending.empty() && batch.size() < limit) {
             Req& q = reqs[pending.front()];
             // with nothing in flight the head request is admitted even if resident state
