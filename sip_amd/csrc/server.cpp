@@ -147,6 +147,10 @@ public:
     // small requests in a batch from which they share one ragged launch (a ragged launch pays a
     // work-table copy)
     int min_ragged = 2;
+    // pending requests from which batches form (0: one more than the slots; measured at 4, 8
+    // and 16 with three slots, the light-load p99s stayed within the run-to-run spread and 16
+    // doubled the 50k req/s p50)
+    int min_queue = 0;
 
     ~Server();
     bool init(std::string* why);
@@ -167,6 +171,7 @@ bool Server::init(std::string* why)
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n > 0) cus = n;
     // ASDFFT_SERVE_MIN_RAGGED=<k>: batches share a ragged launch from k small requests (measurement)
     if (const char* e = getenv("ASDFFT_SERVE_MIN_RAGGED")) min_ragged = std::max(2, atoi(e));
+    if (const char* e = getenv("ASDFFT_SERVE_MIN_QUEUE")) min_queue = std::max(0, atoi(e));
     if (hipStreamCreateWithFlags(&setup, hipStreamNonBlocking) != hipSuccess) {
         *why = "stream creation failed";
         return false;
@@ -497,7 +502,8 @@ void Server::scheduler_loop()
                 if (!(h.ragged && h.bytes <= kRaggedMaxRequestBytes)) break;
                 small_head++;
             }
-            if (small_head < (size_t)min_ragged || pending.size() <= slots.size()) limit = 1;
+            const size_t queue = min_queue > 0 ? (size_t)min_queue : slots.size() + 1;
+            if (small_head < (size_t)min_ragged || pending.size() < queue) limit = 1;
         }
         while (!pending.empty() && batch.size() < limit) {
             Req& q = reqs[pending.front()];
