@@ -45,7 +45,7 @@ struct Req {
     RaggedEntry entry;      // ragged: one entry of `batch` transforms
     std::string plain_key;  // plain: plan cache key
     size_t bytes = 0;
-    Clock::time_point submitted, completed;
+    Clock::time_point submitted, admitted, launched, completed;
     int status = -1;        // -1 in flight, 0 done, 3 device error
 };
 
@@ -465,7 +465,7 @@ void Server::scheduler_loop()
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         int si = -1;
-        cv.wait(lk, [&] {
+        auto ready = [&] {
             if (stop) return true;
             if (pending.empty()) return false;
             for (size_t i = 0; i < slots.size(); i++) {
@@ -475,7 +475,8 @@ void Server::scheduler_loop()
                 }
             }
             return false;
-        });
+        };
+        cv.wait(lk, ready);
         if (stop) break;
         // admission: arrival order while the budget and the slot's tables allow.  The pending
         // requests are shared among the idle slots (each takes its share now, the others take
@@ -537,8 +538,14 @@ void Server::scheduler_loop()
         stats.peak_inflight_bytes = std::max<uint64_t>(stats.peak_inflight_bytes, inflight);
         stats.batches++;
         lk.unlock();
+        const Clock::time_point admitted = Clock::now();
         const int st = launch_batch(s, si, batch);
+        const Clock::time_point launched = Clock::now();
         lk.lock();
+        for (Req* q : batch) {
+            q->admitted = admitted;
+            q->launched = launched;
+        }
         s.error = st;
         s.failed.clear();
         if (st != 0) {
@@ -728,6 +735,9 @@ int serve_replay(Server* s, const ServeRequest* reqs, const double* arrival_us, 
 {
     std::vector<uint64_t> tickets((size_t)n, 0);
     std::vector<Clock::time_point> due((size_t)n);
+    const char* tr = getenv("ASDFFT_SERVE_TRACE");
+    const bool trace = tr != nullptr && tr[0] == '1';
+    std::vector<double> parts[4];
     const Clock::time_point t0 = Clock::now() + std::chrono::milliseconds(1);
     int worst = 0;
     for (int64_t i = 0; i < n; i++) {
@@ -768,8 +778,29 @@ int serve_replay(Server* s, const ServeRequest* reqs, const double* arrival_us, 
         }
         Req& q = it->second;
         latency_us[i] = std::chrono::duration<double, std::micro>(q.completed - due[i]).count();
+        if (trace) {
+            auto us = [](Clock::time_point a, Clock::time_point b) {
+                return std::chrono::duration<double, std::micro>(b - a).count();
+            };
+            parts[0].push_back(us(due[i], q.submitted));
+            parts[1].push_back(us(q.submitted, q.admitted));
+            parts[2].push_back(us(q.admitted, q.launched));
+            parts[3].push_back(us(q.launched, q.completed));
+        }
         if (q.status != 0) worst = std::max(worst, q.status);
         s->reqs.erase(it);
+    }
+    if (trace && !parts[0].empty()) {
+        // ASDFFT_SERVE_TRACE=1: where a replay's latency goes, p50 / p99 of each part
+        static const char* names[4] = {"arrival->submitted", "submitted->admitted", "admitted->launched",
+                                       "launched->completed"};
+        fprintf(stderr, "[asdfft] replay of %lld requests (batching %d):", (long long)n, s->cfg.batching ? 1 : 0);
+        for (int k = 0; k < 4; k++) {
+            std::vector<double>& v = parts[k];
+            std::sort(v.begin(), v.end());
+            fprintf(stderr, " %s %.1f/%.1f", names[k], v[v.size() / 2], v[std::min(v.size() - 1, v.size() * 99 / 100)]);
+        }
+        fprintf(stderr, " us\n");
     }
     return worst;
 }
