@@ -132,6 +132,19 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
 {
     std::vector<int> radices = fitting_radices(n);
     if (radices.empty()) return false;
+    // 4096-point C2C rows with at most one transform per CU: four radix-8 passes on 512 threads
+    // x 8 points instead of three radix-16 passes on 256 x 16, so each CU has twice the waves to
+    // hide one transform's latency (4096 x 256: 10.24 against 10.52-10.56 us per sample; at 512
+    // transforms, two per CU, level: 13.84 against 13.76-13.80).  ASDFFT_WIDE8=0 turns it off.
+    static const int kRadix8x4[4] = {8, 8, 8, 8};
+    const char* w8 = getenv("ASDFFT_WIDE8");
+    const char* nst8 = getenv("ASDFFT_NO_STATIC");  // the geometry exists only as an instance
+    const char* rs8 = getenv("ASDFFT_ROWS_SPLIT");  // the opt-in one-plane rows keep 16x16x16
+    const bool wide8 = !(w8 != nullptr && w8[0] == '0') && !(nst8 != nullptr && nst8[0] == '1') &&
+                       !(rs8 != nullptr && rs8[0] == '1') &&
+                       variant == kVarRowsC2C && want_tpb <= 0 && n == 4096 && count <= kNumCUs &&
+                       find_static(kVarRowsC2C, 16, 4096, 512, 0, kRadix8x4, 4) >= 0;
+    if (wide8) radices.assign(kRadix8x4, kRadix8x4 + 4);
     // Small latency-bound C2C row launches (<= 16 MiB of input) whose passes are all below radix
     // 12: the smallest radix first.  Measured at batch 256: 2187 (3.9.9.9) 7.75 vs 8.21 us, 6000
     // (6.10.10.10) 16.46 vs 17.10, 3000 12.89 vs 13.39, 1215 6.63 vs 6.97; at batch 4096 and for
@@ -194,6 +207,12 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
             best.emax = 16;
             best.threads = (int)t;
         }
+    }
+    if (found && wide8 && best.emax == 16) {
+        best.threads = (int)(n / 8);
+        best.tpb = 1;
+        best.plane = best.tstride;
+        best.lds = (size_t)(8 * best.tstride);
     }
     // ASDFFT_ROWS_SPLIT=1: rows C2C on a specialised one-plane instance for the chosen geometry
     // (half the LDS, twice the resident workgroups).  Opt-in: measured no faster than the

@@ -32,7 +32,7 @@ def test_small_launches_take_the_smallest_radix_first():
     radix first (2187 x 256: 3x9x9x9); larger launches and sequences with radix 12 or 16 keep the
     fewest-pass order."""
     for n, b, want in [(2187, 256, "3x9x9x9"), (2187, 2048, "9x9x9x3"), (6000, 256, "6x10x10x10"),
-                       (1000, 256, "10x10x10"), (4096, 256, "16x16x16"), (1536, 256, "16x16x6")]:
+                       (1000, 256, "10x10x10"), (4096, 512, "16x16x16"), (1536, 256, "16x16x6")]:
         p = Plan("c2c", (n,), b)
         assert field(steps(p.describe())[0], "radices") == want, (n, b)
         p.close()
@@ -293,3 +293,20 @@ def test_streamed_input_on_multi_pass_plans(monkeypatch):
         p = Plan(kind, dims, batch)
         assert "streamed-input" in steps(p.describe())[0], (kind, dims, batch)
         p.close()
+
+
+def test_4096_rows_with_one_transform_per_cu_run_wide(monkeypatch):
+    """4096-point C2C rows with at most one transform per CU run four radix-8 passes on 512
+    threads x 8 points (a specialised instance); two or more per CU keep 16x16x16 on 256 threads.
+    ASDFFT_WIDE8=0 turns it off."""
+    for b, radices, threads in ((256, "8x8x8x8", "512"), (1, "8x8x8x8", "512"), (512, "16x16x16", "256"),
+                                (4096, "16x16x16", "256")):
+        p = Plan("c2c", (4096,), b)
+        s = steps(p.describe())[0]
+        p.close()
+        assert field(s, "radices") == radices and field(s, "T") == threads, (b, s)
+    monkeypatch.setenv("ASDFFT_WIDE8", "0")
+    p = Plan("c2c", (4096,), 256)
+    s = steps(p.describe())[0]
+    p.close()
+    assert field(s, "radices") == "16x16x16", s
