@@ -136,15 +136,24 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
     // x 8 points instead of three radix-16 passes on 256 x 16, so each CU has twice the waves to
     // hide one transform's latency (4096 x 256: 10.24 against 10.52-10.56 us per sample; at 512
     // transforms, two per CU, level: 13.84 against 13.76-13.80).  ASDFFT_WIDE8=0 turns it off.
-    static const int kRadix8x4[4] = {8, 8, 8, 8};
+    struct WideRows {
+        int n, threads;
+        int radices[4];
+    };
+    // (1000 x 256 on 5.5.5.8 with 200 threads measured slower: 7.94 against 7.40-7.44 us)
+    static const WideRows kWide[] = {{4096, 512, {8, 8, 8, 8}}};
     const char* w8 = getenv("ASDFFT_WIDE8");
     const char* nst8 = getenv("ASDFFT_NO_STATIC");  // the geometry exists only as an instance
-    const char* rs8 = getenv("ASDFFT_ROWS_SPLIT");  // the opt-in one-plane rows keep 16x16x16
-    const bool wide8 = !(w8 != nullptr && w8[0] == '0') && !(nst8 != nullptr && nst8[0] == '1') &&
-                       !(rs8 != nullptr && rs8[0] == '1') &&
-                       variant == kVarRowsC2C && want_tpb <= 0 && n == 4096 && count <= kNumCUs &&
-                       find_static(kVarRowsC2C, 16, 4096, 512, 0, kRadix8x4, 4) >= 0;
-    if (wide8) radices.assign(kRadix8x4, kRadix8x4 + 4);
+    const char* rs8 = getenv("ASDFFT_ROWS_SPLIT");  // the opt-in one-plane rows keep their radices
+    const WideRows* wide = nullptr;
+    if (!(w8 != nullptr && w8[0] == '0') && !(nst8 != nullptr && nst8[0] == '1') && !(rs8 != nullptr && rs8[0] == '1') &&
+        variant == kVarRowsC2C && want_tpb <= 0 && count <= kNumCUs) {
+        for (const WideRows& w : kWide) {
+            if (w.n == n && find_static(kVarRowsC2C, 16, w.n, w.threads, 0, w.radices, 4) >= 0) wide = &w;
+        }
+    }
+    const bool wide8 = wide != nullptr;
+    if (wide8) radices.assign(wide->radices, wide->radices + 4);
     // Small latency-bound C2C row launches (<= 16 MiB of input) whose passes are all below radix
     // 12: the smallest radix first.  Measured at batch 256: 2187 (3.9.9.9) 7.75 vs 8.21 us, 6000
     // (6.10.10.10) 16.46 vs 17.10, 3000 12.89 vs 13.39, 1215 6.63 vs 6.97; at batch 4096 and for
@@ -209,7 +218,7 @@ bool rows_config(int64_t n, int64_t count, bool epi_lds, int want_tpb, int varia
         }
     }
     if (found && wide8 && best.emax == 16) {
-        best.threads = (int)(n / 8);
+        best.threads = wide->threads;
         best.tpb = 1;
         best.plane = best.tstride;
         best.lds = (size_t)(8 * best.tstride);
